@@ -1,0 +1,182 @@
+"""Benchmark of the step hot path on one MI355X.
+
+Default workload (N=1): the 100,000-neuron random-cube Izhikevich network
+(~100 Poisson out-synapses each, ~10M synapses), dt 0.5 ms, synaptic tau
+5 ms, Gaussian current sd 1 per step plus the reference's `--noise 1`
+Poisson kicks (1 Hz x 20 mV), int-weights delay ring in Q16.16 (bit-exact
+raster).  One "step" = one simulation step of the whole network; inputs
+(network, state, ring) are resident in HBM before timing starts.
+
+Prints ONE JSON line: synaptic events/s (whole job) with sim-ms/s, the
+roofline of the dominant kernel (algorithmic bytes per launch over its
+HIP-event-timed average launch duration) and the CPU baseline (the oracle,
+one thread, on a bounded sample of the same workload).
+
+    python bench.py [--steps 2000] [--warmup 200] [--n 100000]
+                    [--ring int|float] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
+B_NEURON = 33                   # algorithmic bytes per neuron-step
+B_EVENT = 17                    # algorithmic bytes per synaptic event
+METRIC = "synaptic events/s & sim-ms/s, 100k Izh; batched/served vs plain, same raster"
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def load_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary (if any)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("TRAFFIC_JSON "):
+            return json.loads(line[len("TRAFFIC_JSON "):])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--ring", choices=["int", "float"], default="int")
+    ap.add_argument("--shape", type=int, default=-1, help="-1 auto, 0 two kernels, 1 fused")
+    ap.add_argument("--exec", dest="exec_mode", type=int, default=0)
+    ap.add_argument("--profile-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this backend runs on one GPU; multi-GPU figures are not produced "
+              "(networks are independent units -- see DESIGN.md for the sharding plan)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from neuro_edit_v2_amd import Sim, SimConfig
+    from neuro_edit_v2_amd import net as N
+
+    t_gen = time.time()
+    net = N.cube_net(args.n, mean_deg=100.0, seed=1, w_scale=0.7)
+    t_gen = time.time() - t_gen
+    deg = np.diff(net["src_ptr"].astype(np.int64))
+    dt = 0.5
+    kw = dict(dt_ms=dt, synapse_tau_ms=5.0, noise_sd=1.0, noise_rate_hz=1.0, noise_kick_mv=20.0,
+              seed=1, int_weights=args.ring == "int", int_scale=65536.0)
+    sim = Sim(net, SimConfig(exec_mode=args.exec_mode, step_shape=args.shape, **kw))
+    sim.run_steps(args.warmup)
+    sim.synchronize()
+    ev0, sp0 = sim.events_total(), sim.spikes_total()
+    t0 = time.perf_counter()
+    sim.run_steps(args.steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = sim.last_run_ms()
+    events = sim.events_total() - ev0
+    spikes = sim.spikes_total() - sp0
+    value = events / wall
+
+    # dominant-kernel roofline: per-launch HIP events on the sim's stream
+    ev1 = sim.events_total()
+    main_ms, main_n, aux_ms, aux_n = sim.profile_steps(args.profile_steps)
+    prof_events = sim.events_total() - ev1
+    ev_per_step = prof_events / max(args.profile_steps, 1)
+    from neuro_edit_v2_amd import _lib  # noqa: F401
+    fused = aux_n == 0
+    if fused:
+        kname = "k_neuron (fused step)"
+        kbytes = args.n * B_NEURON + ev_per_step * B_EVENT
+        kavg = main_ms / max(main_n, 1)
+    elif main_ms >= aux_ms:
+        kname = "k_neuron"
+        kbytes = args.n * B_NEURON
+        kavg = main_ms / max(main_n, 1)
+    else:
+        kname = "k_scatter"
+        kbytes = ev_per_step * B_EVENT
+        kavg = aux_ms / max(aux_n, 1)
+    achieved = kbytes / (kavg * 1e-3) / 1e9
+    traffic = load_traffic(f"traffic_{'fused' if fused else 'neuron'}_cube{args.n // 1000}k_{args.ring}.txt")
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "kernel": kname, "avg_launch_us": round(kavg * 1e3, 3),
+            "algorithmic_bytes_per_launch": int(kbytes),
+            "bytes_per_neuron_step": B_NEURON, "bytes_per_event": B_EVENT,
+            "events_per_step": round(ev_per_step, 1),
+            "whole_step_frac": round((args.n * B_NEURON + ev_per_step * B_EVENT)
+                                     / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ref = O.OracleSim(net, O.default_cfg(**kw))
+        warm = min(200, args.cpu_steps)
+        ref.run_steps(warm)
+        c0 = time.perf_counter()
+        rc, ri = ref.run_steps(args.cpu_steps, record=True)
+        cwall = time.perf_counter() - c0
+        cev = int(deg[ri].sum())
+        cpu = {"value": round(cev / cwall, 1), "unit": "synaptic events/s", "cores": 1,
+               "kind": "port",
+               "sample": f"same {args.n}-neuron network and config, {args.cpu_steps} steps after "
+                         f"{warm} warm-up steps, oracle C restatement of CpuSim, 1 thread "
+                         f"({cpu_model()})",
+               "sim_ms_per_s": round(args.cpu_steps * dt / cwall, 2),
+               "wall_s": round(cwall, 3)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state, i32 ring" if args.ring == "int" else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"random-cube Izhikevich {args.n} neurons x Poisson(100) "
+                               f"out-synapses, dt 0.5 ms, 1000 ms per 2000 steps, "
+                               f"{args.ring}-weights ring"
+                               + (" Q16.16" if args.ring == "int" else ""),
+                   "neurons": args.n, "synapses": int(len(net["tgt"])),
+                   "noise": "gaussian sd 1/step + poisson kicks 1 Hz x 20 mV",
+                   "step_shape": "fused" if fused else "two-kernel",
+                   "gen_s": round(t_gen, 2)},
+        "sim_ms_per_s": round(args.steps * dt / wall, 1),
+        "spikes": int(spikes),
+        "events": int(events),
+        "rate_hz": round(spikes / args.n / (args.steps * dt / 1000.0), 3),
+        "gpu_event_ms": round(gpu_ms, 3),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,225 @@
+/*
+ * neuro_hip.h -- C ABI of the MI355X (gfx950) simulation backend.
+ *
+ * This is the drop-in boundary for the reference's `sim` crate backend
+ * trait (sim/src/lib.rs:119-184) and its construction path
+ * (sim/src/lib.rs:205-233).  Every function takes plain pointers and sizes;
+ * outputs are caller-allocated with explicit lengths; no exception or panic
+ * crosses the ABI.  Each function returns an nh_status (0 = ok); the text of
+ * the last failure on the calling thread is nh_last_error().
+ *
+ * Threading: one nh_sim per thread (or serialise calls); each nh_sim owns
+ * one HIP stream.  nh_run_steps is asynchronous; readers synchronise.
+ *
+ * Citations name the reference item each entry point replaces.
+ */
+#ifndef NEURO_HIP_H
+#define NEURO_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NH_OK = 0,
+    NH_ERR_ARG = 1,          /* bad argument / size */
+    NH_ERR_HIP = 2,          /* HIP runtime failure */
+    NH_ERR_NO_DEVICE = 3,    /* no usable GPU */
+    NH_ERR_WINDOW = 4,       /* spike_counts window between ring depth and t */
+    NH_ERR_NONE = 5,         /* Option::None (e.g. spike_row outside window) */
+    NH_ERR_UNSUPPORTED = 6,  /* feature not provided by this backend */
+    NH_ERR_CAPACITY = 7,     /* admission / buffer capacity exceeded */
+    NH_ERR_NET = 8           /* invalid network (e.g. delay past u16 ring) */
+} nh_status;
+
+/* Flat network layout the simulators run: the reference's SynCsr
+ * (net-format/src/network.rs:431-441, built by synapse_csr :654-692),
+ * NeuronParams (sim/src/state.rs:57-74) and StimMap (sim/src/stim.rs:26-31). */
+typedef struct {
+    uint32_t n;
+    uint64_t n_syn;
+    const uint32_t *src_ptr;   /* n+1, outgoing synapses of i: src_ptr[i]..src_ptr[i+1] */
+    const uint32_t *tgt;       /* n_syn */
+    const float *w;            /* n_syn, signed by Dale's law */
+    const uint16_t *delay;     /* n_syn, steps >= 1 */
+    const float *tau;          /* n_syn or NULL; > 0 = custom-tau synapse */
+    uint32_t n_models;         /* model table size (<= 8) */
+    const uint8_t *model_ids;  /* registry id per table entry: 0 izh, 1 adex */
+    const uint8_t *model;      /* n, index into the model table */
+    uint32_t n_params;         /* param row width (max over the table) */
+    uint32_t n_state;          /* state row width (2 for izh and adex) */
+    const float *p;            /* n * n_params */
+    const float *x0;           /* n * n_state */
+    uint32_t n_inputs;
+    const uint32_t *stim_ptr;  /* n+1 or NULL */
+    const uint32_t *stim_input;
+    const float *stim_w;
+} nh_net_desc;
+
+/* SimConfig (sim/src/state.rs:7-49) plus this backend's additions. */
+typedef struct {
+    float dt_ms;
+    float synapse_tau_ms;
+    float readout_tau_ms;
+    float noise_rate_hz;       /* Poisson kicks (reference `--noise`) */
+    float noise_kick_mv;
+    float noise_sd;            /* Gaussian current sd per step (addition) */
+    uint64_t seed;
+    int32_t record_spikes;
+    uint32_t spike_record_steps;
+    int32_t int_weights;       /* fixed-point delay ring (sim/src/kernels.rs:57-66) */
+    float int_scale;           /* 4096 = reference, 65536 = Q16.16 */
+    uint32_t raster_steps;     /* > 0: keep the full spike raster, drained every
+                                  raster_steps steps into host memory (nh_read_spikes) */
+    int32_t exec_mode;         /* 0 auto, 1 direct launches, 2 hipGraph chunks */
+    int32_t step_shape;        /* -1 auto, 0 two kernels per step (neuron, scatter),
+                                  1 one fused kernel per step (delay-1 pulled) */
+    uint32_t graph_steps;      /* steps per captured graph (0 = 64) */
+} nh_config;
+
+/* SimConfig::default() values plus neutral additions. */
+void nh_config_default(nh_config *cfg);
+
+const char *nh_last_error(void);
+
+/* cuda_available -- sim/src/lib.rs:192-203 */
+int nh_available(void);
+
+typedef struct nh_sim nh_sim;
+
+/* Sim::with_params(net, cfg, BackendKind::Cuda, params) -- sim/src/lib.rs:217-233,
+ * CudaSim::with_params -- sim/src/cuda.rs:130-239 */
+int nh_create(const nh_net_desc *net, const nh_config *cfg, nh_sim **out);
+void nh_destroy(nh_sim *s);
+
+/* Backend::run_steps / run_ms -- sim/src/lib.rs:126-127, cuda.rs:584-609 */
+int nh_run_steps(nh_sim *s, uint64_t n_steps);
+int nh_run_ms(nh_sim *s, float ms);
+/* Backend::reset_state -- lib.rs:129, cuda.rs:615-644 */
+int nh_reset(nh_sim *s);
+/* Backend::synchronize -- lib.rs:131 */
+int nh_sync(nh_sim *s);
+/* Backend::n / step_count / time_ms -- lib.rs:123-125 */
+uint32_t nh_n(const nh_sim *s);
+uint64_t nh_step_count(const nh_sim *s);
+double nh_time_ms(const nh_sim *s);
+/* Backend::dev_mem_bytes -- lib.rs:133-135, cuda.rs:804-820 */
+uint64_t nh_dev_mem(const nh_sim *s);
+
+/* Backend::trace -- lib.rs:137 (out: n floats) */
+int nh_read_trace(nh_sim *s, float *out, uint32_t n);
+/* Backend::state(slot) -- lib.rs:139 (out: n floats) */
+int nh_read_state(nh_sim *s, uint32_t slot, float *out, uint32_t n);
+/* Decaying synaptic current s[i] (CpuSim::s, cpu.rs:33); parity probes. */
+int nh_read_syn_current(nh_sim *s, float *out, uint32_t n);
+/* Backend::spikes_total / mean_rate_hz -- lib.rs:149-151 */
+int nh_spikes_total(nh_sim *s, uint64_t *out);
+int nh_mean_rate_hz(nh_sim *s, double *out);
+/* Per-neuron lifetime spike totals (CpuSim::pn_total, cpu.rs:60). */
+int nh_read_pn_total(nh_sim *s, uint64_t *out, uint32_t n);
+/* Synaptic events delivered so far (one per spike x out-synapse). */
+int nh_events_total(nh_sim *s, uint64_t *out);
+/* Backend::spike_row(back) -- lib.rs:148; NH_ERR_NONE for None. */
+int nh_spike_row(nh_sim *s, uint64_t back, uint64_t *words, uint32_t n_words);
+/* Backend::last_spikes -- lib.rs:142 (ascending ids) */
+int nh_last_spikes(nh_sim *s, uint32_t *ids, uint32_t cap, uint32_t *n_out);
+/* Backend::spike_counts(rows) -- lib.rs:146; NH_ERR_WINDOW for the
+ * "exceeds the recorded steps" error (cpu.rs:463-468). */
+int nh_spike_counts(nh_sim *s, uint64_t rows, uint32_t *out, uint32_t n);
+
+/* Backend::set_input_constant / input_values / play_input / loop_input --
+ * lib.rs:152-163 (stimulus program kept host-side as in cuda.rs:241-252) */
+int nh_set_input_constant(nh_sim *s, uint32_t input, float value);
+int nh_input_values(nh_sim *s, float *out, uint32_t n_inputs);
+int nh_play_input(nh_sim *s, uint32_t input, const float *series, uint64_t len);
+int nh_loop_input(nh_sim *s, uint32_t input, const float *series, uint64_t len);
+/* Backend::set_external / set_external_all / external -- lib.rs:158-181 */
+int nh_set_external(nh_sim *s, uint32_t neuron, float amps);
+int nh_set_external_all(nh_sim *s, const float *ext, uint32_t n);
+int nh_read_external(nh_sim *s, float *out, uint32_t n);
+/* Backend::set_noise -- lib.rs:183 */
+int nh_set_noise(nh_sim *s, float rate_hz, float kick_mv);
+/* Gaussian-current sd on a live sim (addition). */
+int nh_set_noise_sd(nh_sim *s, float sd);
+
+/* Backend::set_v_watch / drain_watch -- lib.rs:166-170, cuda.rs:531-570.
+ * drain: out is [state slot][watch][cap_samples]; *n_samples gets the count
+ * per watch (oldest first, at most 4096 kept). */
+int nh_set_v_watch(nh_sim *s, const uint32_t *ids, uint32_t n_ids);
+int nh_drain_watch(nh_sim *s, float *out, uint32_t cap_samples, uint32_t *n_samples);
+
+/* Backend::set_grid_watches / grid_traces -- lib.rs:173-176.  members is
+ * the concatenation of the groups; out is the concatenation of the
+ * per-texel traces. */
+int nh_set_grid_watches(nh_sim *s, uint32_t n_groups, const uint32_t *sizes,
+                        const uint32_t *members);
+int nh_grid_traces(nh_sim *s, float tau_ms, float *out, uint64_t cap);
+
+/* Full spike raster (addition; the reference keeps only a rolling bit
+ * ring).  Drains the host-side raster accumulated since the last call:
+ * counts[k] = spikes at step step0+k, ids = concatenated ids per step in
+ * ascending order.  Returns NH_ERR_CAPACITY (and drains nothing) if the
+ * caller's arrays are too small; *n_steps / *n_ids always get the sizes. */
+int nh_read_spikes(nh_sim *s, uint64_t *step0, uint32_t *counts, uint64_t max_steps,
+                   uint32_t *ids, uint64_t max_ids, uint64_t *n_steps, uint64_t *n_ids);
+
+/* Device timing of the step kernels over the last nh_run_steps call
+ * (hipEvents on the sim's stream): total milliseconds. */
+int nh_last_run_ms(nh_sim *s, double *ms);
+/* Run n_steps with direct launches and a hipEvent pair around every
+ * launch of the step kernels (on the sim's stream).  main_ms / main_n: the
+ * neuron (or fused step) kernel; aux_ms / aux_n: the scatter kernel
+ * (0 in the fused shape).  Same results as nh_run_steps. */
+int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *main_n,
+                     double *aux_ms, uint64_t *aux_n);
+
+/* ---------------- network generators (net-format restated) ----------------
+ * An nh_net owns the flat arrays; nh_net_view fills an nh_net_desc that
+ * points into it. */
+typedef struct nh_net nh_net;
+void nh_net_free(nh_net *net);
+int nh_net_view(const nh_net *net, nh_net_desc *out);
+/* bench::bench_net -- sim/src/bench.rs:10-53 (fixed out-degree k,
+ * constant weights, delays U[1,21) ms exc / 1 ms inh) built at dt. */
+int nh_net_bench(uint32_t n, uint32_t k, uint64_t seed, float w_scale, float w_inh,
+                 float dt_ms, nh_net **out);
+/* Random-cube network: positions U[-500,500)^3, 80% RS / 20% FS,
+ * Poisson(mean_deg) out-degree by inversion, uniform non-self targets,
+ * w = U[0,1)*w_scale (Dale-signed), delay = clamp(1 + floor(19 d / dmax),1,20)
+ * steps with dmax = 1000*sqrt(3). */
+int nh_net_cube(uint32_t n, double mean_deg, uint64_t seed, float w_scale, nh_net **out);
+/* 3D-grid liquid (builders.rs:88-181 semantics with a spatially bucketed
+ * pair loop): edge^3 neurons at spacing 1, p = scale*exp(-(d/lambda)^2),
+ * inh_frac Dale split (set_exc_inh), weight w (izh) -- model 0 izh, 1 adex
+ * (exc TONIC, inh NONADAP).  n_inputs inputs with `per_input` private
+ * uniformly drawn targets of weight w_in. */
+int nh_net_grid(uint32_t ex, uint32_t ey, uint32_t ez, double scale, double lambda,
+                double inh_frac, float w_exc, float w_inh, int model, uint32_t n_inputs,
+                uint32_t per_input, float w_in, uint64_t seed, float dt_ms, nh_net **out);
+
+/* ---------------- batched form ----------------
+ * Many independent networks packed by offset tables into one launch
+ * sequence; each network keeps its own config (dt must match), seed, t
+ * origin and stimulus.  Results are per network and equal the plain
+ * path's (int mode: bit-exact). */
+typedef struct nh_batch nh_batch;
+int nh_batch_create(uint32_t n_nets, const nh_net_desc *nets, const nh_config *cfgs,
+                    nh_batch **out);
+void nh_batch_destroy(nh_batch *b);
+int nh_batch_run_steps(nh_batch *b, uint64_t n_steps);
+int nh_batch_sync(nh_batch *b);
+int nh_batch_spikes_total(nh_batch *b, uint32_t net, uint64_t *out);
+int nh_batch_events_total(nh_batch *b, uint64_t *out);
+int nh_batch_read_state(nh_batch *b, uint32_t net, uint32_t slot, float *out, uint32_t n);
+int nh_batch_read_spikes(nh_batch *b, uint32_t net, uint64_t *step0, uint32_t *counts,
+                         uint64_t max_steps, uint32_t *ids, uint64_t max_ids,
+                         uint64_t *n_steps, uint64_t *n_ids);
+int nh_batch_play_input(nh_batch *b, uint32_t net, uint32_t input, const float *series,
+                        uint64_t len);
+int nh_batch_last_run_ms(nh_batch *b, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,742 @@
+// arena.cpp -- build, launch sequence and readbacks of a packed arena.
+//
+// Construction follows CudaSim::with_params (sim/src/cuda.rs:130-239): the
+// CSR, ring depth (state.rs:221-224), stimulus map and parameter rows are
+// flattened once and uploaded; afterwards the per-step work is enqueued on
+// the arena's stream without host synchronisation (cuda.rs:584-604).
+#include "arena.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+namespace nh {
+
+namespace {
+thread_local std::string g_err;
+}
+
+int set_error(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+const char *last_error_text() { return g_err.c_str(); }
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_error(NH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+Arena::~Arena() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (int i = 0; i < 2; ++i) {
+        if (h_stage[i]) (void)hipHostFree(h_stage[i]);
+        if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Arena::alloc(DBuf &b, size_t bytes) {
+    HIPCHK(b.alloc(bytes));
+    dev_bytes += b.bytes;
+    return NH_OK;
+}
+
+template <class T> int Arena::upload(DBuf &b, const std::vector<T> &v) {
+    int rc = alloc(b, v.size() * sizeof(T));
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return NH_OK;
+}
+
+#define TRY(expr)                \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_) return rc_;     \
+    } while (0)
+
+static uint32_t align64(uint64_t v) { return (uint32_t)((v + 63) & ~uint64_t(63)); }
+
+// sim/src/kernels.rs:61 -- (int)lrintf(w * WSCALE), round half to even.
+static int32_t quantize(float w, float scale) { return (int32_t)std::lrintf(w * scale); }
+
+int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, uint32_t raster_steps,
+                 const Tuning &tune) {
+    if (nn == 0) return set_error(NH_ERR_ARG, "no networks");
+    if (nn > 65535) return set_error(NH_ERR_ARG, "at most 65535 networks per arena");
+    n_nets = nn;
+    ring_int = cfgs[0].int_weights ? 1 : 0;
+    slots.resize(nn);
+    // ---- validate and lay out
+    uint64_t off = 0;
+    uint32_t maxd = 1;
+    bool any_izh = false, any_adex = false;
+    uint32_t P = 4;
+    uint64_t total_syn = 0;
+    uint32_t cols = 0;
+    for (uint32_t k = 0; k < nn; ++k) {
+        const nh_net_desc &d = nets[k];
+        if ((cfgs[k].int_weights ? 1 : 0) != ring_int)
+            return set_error(NH_ERR_ARG, "all packed networks must share int_weights");
+        if (!(cfgs[k].dt_ms > 0.0f) || !std::isfinite(cfgs[k].dt_ms))
+            return set_error(NH_ERR_ARG, "dt_ms must be positive");
+        if (d.n && (!d.src_ptr || !d.model || !d.p || !d.x0))
+            return set_error(NH_ERR_ARG, "network arrays missing");
+        if (d.n_models == 0 && d.n) return set_error(NH_ERR_ARG, "empty model table");
+        if (d.n_models > 8) return set_error(NH_ERR_ARG, "model table too large");
+        if (d.n && d.n_state != 2) return set_error(NH_ERR_ARG, "state row width must be 2");
+        if (d.n_params > (uint32_t)kMaxParams) return set_error(NH_ERR_ARG, "param row too wide");
+        for (uint32_t m = 0; m < d.n_models; ++m) {
+            if (d.model_ids[m] > 1)
+                return set_error(NH_ERR_ARG, "unknown model id " + std::to_string(d.model_ids[m]));
+        }
+        for (uint32_t i = 0; i < d.n; ++i) {
+            if (d.model[i] >= d.n_models)
+                return set_error(NH_ERR_ARG, "neuron " + std::to_string(i) +
+                                                 " names model " + std::to_string(d.model[i]) +
+                                                 " outside the model table");
+            const uint8_t id = d.model_ids[d.model[i]];
+            if (id == 0) any_izh = true;
+            else any_adex = true;
+            if (id == 1 && d.n_params < 10)
+                return set_error(NH_ERR_ARG, "adex rows need 10 params");
+            if (id == 0 && d.n_params < 4) return set_error(NH_ERR_ARG, "izh rows need 4 params");
+        }
+        if (d.n && d.src_ptr[0] != 0) return set_error(NH_ERR_ARG, "src_ptr[0] must be 0");
+        for (uint32_t i = 0; i < d.n; ++i)
+            if (d.src_ptr[i + 1] < d.src_ptr[i]) return set_error(NH_ERR_ARG, "src_ptr not monotone");
+        const uint64_t E = d.n ? d.src_ptr[d.n] : 0;
+        if (E != d.n_syn) return set_error(NH_ERR_ARG, "src_ptr[n] != n_syn");
+        for (uint64_t j = 0; j < E; ++j) {
+            if (d.tgt[j] >= d.n) return set_error(NH_ERR_ARG, "synapse target out of range");
+            if (d.delay[j] < 1) return set_error(NH_ERR_ARG, "delay must be >= 1 step");
+            if (d.tau && d.tau[j] > 0.0f)
+                return set_error(NH_ERR_UNSUPPORTED,
+                                 "custom-tau synapses (tau_ms > 0) are not supported by the HIP "
+                                 "backend");
+            maxd = std::max<uint32_t>(maxd, d.delay[j]);
+        }
+        if (d.n_inputs && d.stim_ptr) {
+            for (uint32_t i = 0; i < d.n; ++i)
+                for (uint32_t j = d.stim_ptr[i]; j < d.stim_ptr[i + 1]; ++j)
+                    if (d.stim_input[j] >= d.n_inputs)
+                        return set_error(NH_ERR_ARG, "stimulus input index out of range");
+        }
+        P = std::max(P, d.n_params);
+        total_syn += E;
+        slots[k].base = (uint32_t)off;
+        slots[k].n = d.n;
+        slots[k].cfg = cfgs[k];
+        slots[k].n_inputs = d.n_inputs;
+        slots[k].stim_base = cols;
+        cols += d.n_inputs;
+        off += align64(d.n);
+        if (off > 0x7FFFFFFFull) return set_error(NH_ERR_CAPACITY, "arena exceeds 2^31 neurons");
+    }
+    n_total = (uint32_t)off;
+    if (n_total == 0) n_total = 0;
+    words = n_total / 64;
+    D = std::max<uint32_t>(maxd + 1, 2);  // ring_depth, state.rs:221-224
+    n_syn = total_syn;
+    mmode = any_adex ? (any_izh ? 2 : 1) : 0;
+    stim_cols = cols;
+    packed = (n_total <= (1u << 27) && D <= 32) ? 1 : 0;
+
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+
+    // ---- CSR over arena ids, rows sorted by (delay, target)
+    std::vector<uint32_t> src_ptr(n_total + 1, 0);
+    std::vector<uint32_t> syn(std::max<uint64_t>(total_syn, 1));
+    std::vector<uint16_t> d16;
+    if (!packed) d16.resize(std::max<uint64_t>(total_syn, 1));
+    std::vector<float> wf;
+    std::vector<int32_t> wq;
+    if (ring_int) wq.resize(std::max<uint64_t>(total_syn, 1));
+    else wf.resize(std::max<uint64_t>(total_syn, 1));
+    std::vector<uint32_t> skip1(n_total + 1, 0);
+    std::vector<uint32_t> in1_cnt(n_total + 1, 0);
+    {
+        uint64_t o = 0;
+        std::vector<uint32_t> idx;
+        for (uint32_t k = 0; k < nn; ++k) {
+            const nh_net_desc &d = nets[k];
+            const uint32_t b = slots[k].base;
+            const float scale = cfgs[k].int_scale;
+            for (uint32_t i = 0; i < d.n; ++i) {
+                const uint32_t lo = d.src_ptr[i], hi = d.src_ptr[i + 1];
+                src_ptr[b + i] = (uint32_t)o;
+                idx.resize(hi - lo);
+                std::iota(idx.begin(), idx.end(), lo);
+                std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
+                    if (d.delay[x] != d.delay[y]) return d.delay[x] < d.delay[y];
+                    return d.tgt[x] < d.tgt[y];
+                });
+                uint32_t s1 = 0;
+                for (uint32_t j : idx) {
+                    const uint32_t tg = b + d.tgt[j];
+                    if (packed) syn[o] = tg | ((uint32_t)d.delay[j] << 27);
+                    else {
+                        syn[o] = tg;
+                        d16[o] = d.delay[j];
+                    }
+                    if (ring_int) wq[o] = quantize(d.w[j], scale);
+                    else wf[o] = d.w[j];
+                    if (d.delay[j] == 1) {
+                        ++s1;
+                        in1_cnt[tg + 1]++;
+                    }
+                    ++o;
+                }
+                skip1[b + i] = s1;
+            }
+            // padding neurons of this network have empty rows
+            for (uint32_t i = d.n; i < align64(d.n); ++i) src_ptr[b + i] = (uint32_t)o;
+        }
+        src_ptr[n_total] = (uint32_t)o;
+        if (o > 0xFFFFFFFFull) return set_error(NH_ERR_CAPACITY, "more than 2^32 synapses");
+    }
+    // delay-1 in-lists (target-major), sources ascending
+    std::vector<uint32_t> in1_ptr(n_total + 1, 0);
+    for (uint32_t i = 0; i < n_total; ++i) in1_ptr[i + 1] = in1_ptr[i] + in1_cnt[i + 1];
+    n_in1 = in1_ptr[n_total];
+    std::vector<uint32_t> in1_src(std::max<uint64_t>(n_in1, 1));
+    std::vector<float> in1_wf;
+    std::vector<int32_t> in1_wq;
+    if (ring_int) in1_wq.resize(std::max<uint64_t>(n_in1, 1));
+    else in1_wf.resize(std::max<uint64_t>(n_in1, 1));
+    {
+        std::vector<uint32_t> fill(n_total, 0);
+        for (uint32_t s = 0; s < n_total; ++s) {
+            const uint32_t lo = src_ptr[s], n1 = skip1[s];
+            for (uint32_t j = lo; j < lo + n1; ++j) {
+                const uint32_t tg = packed ? (syn[j] & 0x07FFFFFFu) : syn[j];
+                const uint32_t o = in1_ptr[tg] + fill[tg]++;
+                in1_src[o] = s;
+                if (ring_int) in1_wq[o] = wq[j];
+                else in1_wf[o] = wf[j];
+            }
+        }
+    }
+    // fused shape when the pull is cheap next to the neuron update
+    if (tune.fused >= 0) fused = tune.fused;
+    else fused = (n_in1 <= (uint64_t)n_total / 2) ? 1 : 0;
+
+    // ---- parameters: type table when the arena has <= 256 distinct rows
+    std::map<std::vector<uint32_t>, uint32_t> rows;
+    std::vector<uint8_t> prow(n_total, 0);
+    std::vector<PRow> ptab;
+    bool table_ok = true;
+    for (uint32_t k = 0; k < nn && table_ok; ++k) {
+        const nh_net_desc &d = nets[k];
+        for (uint32_t i = 0; i < d.n; ++i) {
+            std::vector<uint32_t> key(kMaxParams + 1, 0);
+            key[0] = d.model_ids[d.model[i]];
+            for (uint32_t q = 0; q < d.n_params; ++q)
+                std::memcpy(&key[1 + q], &d.p[(uint64_t)i * d.n_params + q], 4);
+            auto it = rows.find(key);
+            uint32_t r;
+            if (it == rows.end()) {
+                if (rows.size() >= 256) {
+                    table_ok = false;
+                    break;
+                }
+                r = (uint32_t)rows.size();
+                rows.emplace(key, r);
+                PRow pr{};
+                for (int q = 0; q < kMaxParams; ++q) std::memcpy(&pr.p[q], &key[1 + q], 4);
+                pr.model = key[0];
+                ptab.push_back(pr);
+            } else {
+                r = it->second;
+            }
+            prow[slots[k].base + i] = (uint8_t)r;
+        }
+    }
+    pmode = table_ok ? 0 : 1;
+    std::vector<float> pflat;
+    std::vector<uint8_t> mreg;
+    if (pmode == 1) {
+        pflat.assign((uint64_t)n_total * P, 0.0f);
+        mreg.assign(n_total, 0);
+        for (uint32_t k = 0; k < nn; ++k) {
+            const nh_net_desc &d = nets[k];
+            for (uint32_t i = 0; i < d.n; ++i) {
+                const uint32_t a = slots[k].base + i;
+                mreg[a] = d.model_ids[d.model[i]];
+                for (uint32_t q = 0; q < d.n_params; ++q)
+                    pflat[(uint64_t)a * P + q] = d.p[(uint64_t)i * d.n_params + q];
+            }
+        }
+        prow.clear();
+    }
+    this->P_ = P;
+
+    // ---- per-network records, per-wave network index
+    recs.resize(nn);
+    std::vector<uint16_t> wave_net(std::max<uint32_t>(words, 1), 0);
+    for (uint32_t k = 0; k < nn; ++k) {
+        const nh_config &c = cfgs[k];
+        NetRec &r = recs[k];
+        std::memset(&r, 0, sizeof(r));
+        r.base = slots[k].base;
+        r.n = slots[k].n;
+        r.seed = c.seed;
+        r.t0 = 0;
+        fill_noise(r, c);
+        r.int_scale = c.int_scale;
+        r.stim_base = slots[k].stim_base;
+        r.active = 1;
+        r.dt = c.dt_ms;
+        for (uint32_t w = r.base / 64; w < (r.base + align64(r.n)) / 64; ++w) wave_net[w] = (uint16_t)k;
+    }
+
+    // ---- initial state and stimulus map over arena ids
+    std::vector<float2> x0(std::max<uint32_t>(n_total, 1), make_float2(0.f, 0.f));
+    std::vector<uint32_t> stim_ptr;
+    std::vector<uint32_t> stim_input;
+    std::vector<float> stim_w;
+    bool any_stim = false;
+    for (uint32_t k = 0; k < nn; ++k)
+        if (nets[k].n_inputs && nets[k].stim_ptr && nets[k].stim_ptr[nets[k].n]) any_stim = true;
+    if (any_stim) stim_ptr.assign(n_total + 1, 0);
+    for (uint32_t k = 0; k < nn; ++k) {
+        const nh_net_desc &d = nets[k];
+        const uint32_t b = slots[k].base;
+        for (uint32_t i = 0; i < d.n; ++i)
+            x0[b + i] = make_float2(d.x0[(uint64_t)i * 2], d.x0[(uint64_t)i * 2 + 1]);
+        if (!any_stim) continue;
+        for (uint32_t i = 0; i < align64(d.n); ++i) {
+            stim_ptr[b + i] = (uint32_t)stim_input.size();
+            if (i < d.n && d.n_inputs && d.stim_ptr)
+                for (uint32_t j = d.stim_ptr[i]; j < d.stim_ptr[i + 1]; ++j) {
+                    stim_input.push_back(d.stim_input[j]);
+                    stim_w.push_back(d.stim_w[j]);
+                }
+        }
+    }
+    if (any_stim) stim_ptr[n_total] = (uint32_t)stim_input.size();
+
+    // ---- device buffers
+    TRY(alloc(d_t, 8));
+    TRY(upload(d_nets, recs));
+    TRY(upload(d_wave_net, wave_net));
+    TRY(upload(d_x0, x0));
+    TRY(upload(d_x, x0));
+    TRY(alloc(d_s, (size_t)n_total * 4));
+    TRY(alloc(d_ring, (size_t)D * n_total * 4));
+    TRY(alloc(d_ext, (size_t)n_total * 4));
+    if (pmode == 0) {
+        TRY(upload(d_prow, prow));
+        TRY(upload(d_ptab, ptab));
+    } else {
+        TRY(upload(d_p, pflat));
+        TRY(upload(d_mreg, mreg));
+    }
+    if (any_stim) {
+        TRY(upload(d_stim_ptr, stim_ptr));
+        TRY(upload(d_stim_input, stim_input));
+        TRY(upload(d_stim_w, stim_w));
+    }
+    stim_cap = 1024;
+    if (stim_cols) {
+        TRY(alloc(d_stim_win, (size_t)stim_cap * stim_cols * 4));
+        for (int i = 0; i < 2; ++i) {
+            HIPCHK(hipHostMalloc((void **)&h_stage[i], (size_t)stim_cap * stim_cols * 4, 0));
+            HIPCHK(hipEventCreateWithFlags(&stage_ev[i], hipEventDisableTiming));
+        }
+    }
+    if (fused) {
+        TRY(upload(d_in1_ptr, in1_ptr));
+        TRY(upload(d_in1_src, in1_src));
+        if (ring_int) TRY(upload(d_in1_w, in1_wq));
+        else TRY(upload(d_in1_w, in1_wf));
+        TRY(upload(d_skip1, skip1));
+    }
+    TRY(alloc(d_pbits, (size_t)2 * std::max<uint32_t>(words, 1) * 8));
+    TRY(alloc(d_spk, (size_t)3 * std::max<uint32_t>(n_total, 1) * 4));
+    TRY(alloc(d_cnt, 4 * 4));
+    bits_rows = 0;
+    if (nn == 1 && cfgs[0].record_spikes) {
+        bits_rows = std::max<uint32_t>(cfgs[0].spike_record_steps, 2);
+        TRY(alloc(d_bits, (size_t)bits_rows * std::max<uint32_t>(words, 1) * 8));
+    }
+    TRY(alloc(d_pn, (size_t)n_total * 4));
+    TRY(alloc(d_last, (size_t)n_total * 4));
+    HIPCHK(hipMemset(d_last.p, 0xFF, d_last.bytes));
+    TRY(alloc(d_net_spikes, (size_t)nn * 8));
+    TRY(upload(d_src_ptr, src_ptr));
+    TRY(upload(d_syn, syn));
+    if (!packed) TRY(upload(d_delay16, d16));
+    if (ring_int) TRY(upload(d_w, wq));
+    else TRY(upload(d_w, wf));
+    TRY(alloc(d_totals, 8 * 8));
+    rast_steps = raster_steps;
+    if (rast_steps) {
+        rast_cap_ids = std::min<uint64_t>((uint64_t)std::max<uint32_t>(n_total, 1) * rast_steps,
+                                          1ull << 28);
+        TRY(alloc(d_rast_ids, rast_cap_ids * 4));
+        TRY(alloc(d_rast_off, (size_t)rast_steps * 4));
+    }
+    TRY(alloc(d_rast_ctl, 4 * 8));
+    h_src_ptr_ = std::move(src_ptr);
+    HIPCHK(hipDeviceSynchronize());
+
+    use_graph = tune.use_graph >= 0 ? tune.use_graph : 1;
+    graph_steps = tune.graph_steps ? tune.graph_steps : 64;
+    scatter_blocks = tune.scatter_blocks;
+    return NH_OK;
+}
+
+void Arena::fill_noise(NetRec &r, const nh_config &c) {
+    // sim/src/cpu.rs:197-204 (host-side scalars, f32)
+    r.pkick = c.noise_rate_hz * c.dt_ms / 1000.0f;
+    r.kick = c.noise_kick_mv != 0.0f ? c.noise_kick_mv / c.dt_ms : 0.0f;
+    r.sd = c.noise_sd;
+    const float tau = c.synapse_tau_ms > 1e-6f ? c.synapse_tau_ms : 1e-6f;
+    r.decay_s = std::exp(-c.dt_ms / tau);
+}
+
+int Arena::upload_recs() {
+    HIPCHK(hipMemcpyAsync(d_nets.p, recs.data(), recs.size() * sizeof(NetRec),
+                          hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return NH_OK;
+}
+
+StepArgs Arena::args() const {
+    StepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n_total = n_total;
+    a.D = D;
+    a.t_dev = d_t.as<uint64_t>();
+    a.k = 0;
+    a.single_net = n_nets == 1 ? 1 : 0;
+    a.wave_net = d_wave_net.as<uint16_t>();
+    a.nets = d_nets.as<NetRec>();
+    a.x = d_x.as<float2>();
+    a.s = d_s.as<float>();
+    a.ring = d_ring.p;
+    a.ext = has_ext ? d_ext.as<float>() : nullptr;
+    a.prow = d_prow.as<uint8_t>();
+    a.ptab = d_ptab.as<PRow>();
+    a.p = d_p.as<float>();
+    a.mreg = d_mreg.as<uint8_t>();
+    a.P = P_;
+    a.stim_ptr = d_stim_ptr.as<uint32_t>();
+    a.stim_input = d_stim_input.as<uint32_t>();
+    a.stim_w = d_stim_w.as<float>();
+    a.stim_win = d_stim_win.as<float>();
+    a.stim_cols = stim_cols;
+    a.in1_ptr = d_in1_ptr.as<uint32_t>();
+    a.in1_src = d_in1_src.as<uint32_t>();
+    a.in1_w = d_in1_w.p;
+    a.pbits = fused ? d_pbits.as<uint64_t>() : nullptr;
+    a.spk = d_spk.as<uint32_t>();
+    a.cnt = d_cnt.as<uint32_t>();
+    a.bits = bits_rows ? d_bits.as<uint64_t>() : nullptr;
+    a.bits_rows = bits_rows ? bits_rows : 1;
+    a.words = words;
+    a.pn_total = d_pn.as<uint32_t>();
+    a.last_spike = d_last.as<uint32_t>();
+    a.net_spikes = n_nets > 1 ? d_net_spikes.as<uint64_t>() : nullptr;
+    a.rast_ids = rast_steps ? d_rast_ids.as<uint32_t>() : nullptr;
+    a.rast_off = rast_steps ? d_rast_off.as<uint32_t>() : nullptr;
+    a.rast_ctl = d_rast_ctl.as<uint64_t>();
+    a.rast_cap_ids = rast_cap_ids;
+    a.rast_cap_steps = rast_steps;
+    a.src_ptr = d_src_ptr.as<uint32_t>();
+    a.syn = d_syn.as<uint32_t>();
+    a.delay16 = d_delay16.as<uint16_t>();
+    a.w = d_w.p;
+    a.skip1 = d_skip1.as<uint32_t>();
+    a.totals = d_totals.as<uint64_t>();
+    a.watch = d_watch.as<uint32_t>();
+    a.n_watch = n_watch;
+    a.xrec = d_xrec.as<float2>();
+    a.xrec_depth = xrec_depth;
+    return a;
+}
+
+uint32_t Arena::neuron_blocks() const {
+    return std::max<uint32_t>((n_total + kBlock - 1) / kBlock, 1);
+}
+
+uint32_t Arena::scat_blocks() const {
+    if (scatter_blocks) return scatter_blocks;
+    return std::min<uint32_t>(std::max<uint32_t>(neuron_blocks(), 1), 2048);
+}
+
+int Arena::launch_chunk(uint32_t k) {
+    StepArgs a = args();
+    const uint32_t nb = neuron_blocks(), sb = scat_blocks();
+    for (uint32_t j = 0; j < k; ++j) {
+        a.k = j;
+        launch_neuron(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
+        if (!fused) launch_scatter(a, sb, ring_int, packed != 0, stream);
+        if (n_watch) launch_watch(a, stream);
+    }
+    launch_advance(d_t.as<uint64_t>(), k, stream);
+    HIPCHK(hipGetLastError());
+    return NH_OK;
+}
+
+int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,
+                   uint64_t &aux_n) {
+    main_ms = aux_ms = 0.0;
+    main_n = aux_n = 0;
+    if (n_total == 0 || n_steps == 0) {
+        t += n_steps;
+        return NH_OK;
+    }
+    constexpr uint32_t kPool = 64;  // steps per event batch
+    std::vector<hipEvent_t> ev(4 * kPool, nullptr);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = NH_OK;
+    const uint32_t nb = neuron_blocks(), sb = scat_blocks();
+    while (n_steps && rc == NH_OK) {
+        uint64_t k = std::min<uint64_t>(n_steps, std::min<uint32_t>(kPool, stim_cap));
+        if (rast_steps) {
+            const uint64_t room = rast_steps - (t - rast_step0);
+            if (room == 0) {
+                rc = drain_raster();
+                continue;
+            }
+            k = std::min<uint64_t>(k, room);
+        }
+        if ((rc = upload_stim(t, (uint32_t)k))) break;
+        StepArgs a = args();
+        for (uint32_t j = 0; j < k; ++j) {
+            a.k = j;
+            (void)hipEventRecord(ev[4 * j], stream);
+            launch_neuron(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
+            (void)hipEventRecord(ev[4 * j + 1], stream);
+            if (!fused) {
+                (void)hipEventRecord(ev[4 * j + 2], stream);
+                launch_scatter(a, sb, ring_int, packed != 0, stream);
+                (void)hipEventRecord(ev[4 * j + 3], stream);
+            }
+            if (n_watch) launch_watch(a, stream);
+        }
+        launch_advance(d_t.as<uint64_t>(), (uint32_t)k, stream);
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            rc = set_error(NH_ERR_HIP, std::string("profile: ") + hipGetErrorString(e));
+            break;
+        }
+        for (uint32_t j = 0; j < k; ++j) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[4 * j], ev[4 * j + 1]) == hipSuccess) {
+                main_ms += ms;
+                ++main_n;
+            }
+            if (!fused && hipEventElapsedTime(&ms, ev[4 * j + 2], ev[4 * j + 3]) == hipSuccess) {
+                aux_ms += ms;
+                ++aux_n;
+            }
+        }
+        t += k;
+        n_steps -= k;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int Arena::capture_graph() {
+    if (graph_exec) {
+        (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+    }
+    if (graph) {
+        (void)hipGraphDestroy(graph);
+        graph = nullptr;
+    }
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_chunk(graph_steps);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_error(NH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    graph = g;
+    HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    graph_dirty = false;
+    return NH_OK;
+}
+
+int Arena::upload_stim(uint64_t t_arena, uint32_t k) {
+    if (!stim_cols || !packer_) return NH_OK;
+    float *h = h_stage[stage_i];
+    if (stage_used[stage_i]) HIPCHK(hipEventSynchronize(stage_ev[stage_i]));
+    stage_used[stage_i] = true;
+    packer_(packer_ctx_, t_arena, k, h);
+    HIPCHK(hipMemcpyAsync(d_stim_win.p, h, (size_t)k * stim_cols * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipEventRecord(stage_ev[stage_i], stream));
+    stage_i ^= 1;
+    return NH_OK;
+}
+
+int Arena::run(uint64_t n_steps) {
+    if (n_total == 0 || n_steps == 0) {
+        t += n_steps;
+        last_run_ms = 0.0;
+        return NH_OK;
+    }
+    if (use_graph && graph_dirty) TRY(capture_graph());
+    HIPCHK(hipEventRecord(ev0, stream));
+    while (n_steps) {
+        uint64_t k = std::min<uint64_t>(n_steps, use_graph ? graph_steps : stim_cap);
+        k = std::min<uint64_t>(k, stim_cap);
+        if (rast_steps) {
+            const uint64_t room = rast_steps - (t - rast_step0);
+            if (room == 0) {
+                TRY(drain_raster());
+                continue;
+            }
+            k = std::min<uint64_t>(k, room);
+        }
+        TRY(upload_stim(t, (uint32_t)k));
+        if (use_graph && k == graph_steps) HIPCHK(hipGraphLaunch(graph_exec, stream));
+        else TRY(launch_chunk((uint32_t)k));
+        t += k;
+        n_steps -= k;
+    }
+    HIPCHK(hipEventRecord(ev1, stream));
+    timing_valid = true;
+    return NH_OK;
+}
+
+int Arena::sync() {
+    HIPCHK(hipStreamSynchronize(stream));
+    if (timing_valid) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_run_ms = ms;
+        timing_valid = false;
+    }
+    return NH_OK;
+}
+
+int Arena::reset() {
+    TRY(sync());
+    HIPCHK(hipMemcpyAsync(d_x.p, d_x0.p, d_x0.bytes, hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipMemsetAsync(d_s.p, 0, d_s.bytes, stream));
+    HIPCHK(hipMemsetAsync(d_ring.p, 0, d_ring.bytes, stream));
+    HIPCHK(hipMemsetAsync(d_pbits.p, 0, d_pbits.bytes, stream));
+    HIPCHK(hipMemsetAsync(d_cnt.p, 0, d_cnt.bytes, stream));
+    HIPCHK(hipMemsetAsync(d_totals.p, 0, d_totals.bytes, stream));
+    if (d_bits.p) HIPCHK(hipMemsetAsync(d_bits.p, 0, d_bits.bytes, stream));
+    HIPCHK(hipMemsetAsync(d_pn.p, 0, d_pn.bytes, stream));
+    HIPCHK(hipMemsetAsync(d_last.p, 0xFF, d_last.bytes, stream));
+    HIPCHK(hipMemsetAsync(d_net_spikes.p, 0, d_net_spikes.bytes, stream));
+    HIPCHK(hipMemsetAsync(d_rast_ctl.p, 0, d_rast_ctl.bytes, stream));
+    HIPCHK(hipMemsetAsync(d_t.p, 0, 8, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    t = 0;
+    rast_id0 = rast_step0 = 0;
+    h_rast_cnt.clear();
+    h_rast_ids.clear();
+    h_rast_step0 = 0;
+    return NH_OK;
+}
+
+int Arena::read_x(std::vector<float2> &out) {
+    TRY(sync());
+    out.resize(n_total);
+    if (n_total) HIPCHK(hipMemcpy(out.data(), d_x.p, (size_t)n_total * 8, hipMemcpyDeviceToHost));
+    return NH_OK;
+}
+
+int Arena::read_f32(const DBuf &b, std::vector<float> &out, size_t n) {
+    TRY(sync());
+    out.resize(n);
+    if (n) HIPCHK(hipMemcpy(out.data(), b.p, n * 4, hipMemcpyDeviceToHost));
+    return NH_OK;
+}
+
+int Arena::read_u32(const DBuf &b, std::vector<uint32_t> &out, size_t n) {
+    TRY(sync());
+    out.resize(n);
+    if (n) HIPCHK(hipMemcpy(out.data(), b.p, n * 4, hipMemcpyDeviceToHost));
+    return NH_OK;
+}
+
+int Arena::read_u64(const DBuf &b, std::vector<uint64_t> &out, size_t n) {
+    TRY(sync());
+    out.resize(n);
+    if (n) HIPCHK(hipMemcpy(out.data(), b.p, n * 8, hipMemcpyDeviceToHost));
+    return NH_OK;
+}
+
+int Arena::spikes_total(uint64_t &out) {
+    out = 0;
+    if (t == 0 || n_total == 0) return NH_OK;
+    std::vector<uint64_t> tot;
+    std::vector<uint32_t> cnt;
+    TRY(read_u64(d_totals, tot, 8));
+    TRY(read_u32(d_cnt, cnt, 4));
+    const uint32_t L = (uint32_t)((t - 1) % 3);
+    out = tot[2 + L] + cnt[L];
+    return NH_OK;
+}
+
+int Arena::events_total(uint64_t &out) {
+    out = 0;
+    if (t == 0 || n_total == 0) return NH_OK;
+    std::vector<uint64_t> tot;
+    TRY(read_u64(d_totals, tot, 8));
+    out = tot[1];
+    if (fused) {
+        // the last step's spikes are pushed by the next step's launch
+        std::vector<uint32_t> cnt;
+        TRY(read_u32(d_cnt, cnt, 4));
+        const uint32_t L = (uint32_t)((t - 1) % 3);
+        const uint32_t c = cnt[L];
+        if (c) {
+            std::vector<uint32_t> ids(c);
+            HIPCHK(hipMemcpy(ids.data(), d_spk.as<uint32_t>() + (size_t)L * n_total, (size_t)c * 4,
+                             hipMemcpyDeviceToHost));
+            for (uint32_t i : ids) out += h_src_ptr_[i + 1] - h_src_ptr_[i];
+        }
+    }
+    return NH_OK;
+}
+
+int Arena::drain_raster() {
+    if (!rast_steps) return NH_OK;
+    uint64_t total = 0;
+    TRY(spikes_total(total));
+    std::vector<uint64_t> ctl;
+    TRY(read_u64(d_rast_ctl, ctl, 4));
+    if (ctl[2]) return set_error(NH_ERR_CAPACITY, "raster overflow: lower raster_steps");
+    const uint64_t steps = t - rast_step0;
+    const uint64_t nid = total - rast_id0;
+    if (steps) {
+        std::vector<uint32_t> offs(steps);
+        HIPCHK(hipMemcpy(offs.data(), d_rast_off.p, steps * 4, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> ids(nid);
+        if (nid) HIPCHK(hipMemcpy(ids.data(), d_rast_ids.p, nid * 4, hipMemcpyDeviceToHost));
+        if (h_rast_cnt.empty()) h_rast_step0 = rast_step0;
+        for (uint64_t s = 0; s < steps; ++s) {
+            const uint64_t a = offs[s], b = s + 1 < steps ? offs[s + 1] : nid;
+            if (b < a || b > nid) return set_error(NH_ERR_HIP, "raster offsets inconsistent");
+            std::sort(ids.begin() + a, ids.begin() + b);
+            h_rast_cnt.push_back((uint32_t)(b - a));
+        }
+        h_rast_ids.insert(h_rast_ids.end(), ids.begin(), ids.end());
+    }
+    rast_id0 = total;
+    rast_step0 = t;
+    const uint64_t nc[3] = {rast_id0, rast_step0, 0};
+    HIPCHK(hipMemcpy(d_rast_ctl.p, nc, sizeof(nc), hipMemcpyHostToDevice));
+    return NH_OK;
+}
+
+}  // namespace nh

@@ -1,0 +1,154 @@
+// arena.h -- host side of the step engine: device buffers of one arena
+// (one network, or many packed by offset tables), the per-chunk launch
+// sequence (direct launches or a captured hipGraph), and readbacks.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/neuro_hip.h"
+#include "layout.h"
+
+namespace nh {
+
+void launch_neuron(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool fused,
+                   bool packed, hipStream_t st);
+void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st);
+void launch_watch(const StepArgs &a, hipStream_t st);
+void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
+
+int set_error(int code, const std::string &msg);
+
+// Owned device allocation.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        bytes = n ? n : 4;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            bytes = 0;
+            return e;
+        }
+        return hipMemset(p, 0, bytes);
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+// One packed network as the arena sees it.
+struct NetSlot {
+    uint32_t base = 0, n = 0;
+    uint32_t n_inputs = 0, stim_base = 0;
+    nh_config cfg{};
+};
+
+struct Tuning {
+    int fused = -1;          // -1 auto, 0 two-phase, 1 fused
+    int use_graph = -1;      // -1 auto
+    uint32_t graph_steps = 64;
+    uint32_t scatter_blocks = 0;  // 0 auto
+};
+
+class Arena {
+  public:
+    ~Arena();
+    // Build from n networks.  All networks share int_weights; the ring
+    // depth is the max of theirs (a deeper ring leaves results unchanged).
+    int build(uint32_t n_nets, const nh_net_desc *nets, const nh_config *cfgs, uint32_t raster_steps,
+              const Tuning &tune);
+    int reset();
+    // Run n steps asynchronously (stimulus windows packed by the packer).
+    int run(uint64_t n_steps);
+    int sync();
+    int upload_recs();
+    static void fill_noise(NetRec &r, const nh_config &c);
+    uint32_t neuron_blocks() const;
+    uint32_t scat_blocks() const;
+    int upload_stim(uint64_t t_arena, uint32_t k);
+
+    // Stimulus: the caller supplies a packer producing [k][stim_cols].
+    using StimPacker = void (*)(void *ctx, uint64_t t_arena, uint32_t k, float *out);
+    void set_stim_packer(StimPacker p, void *ctx) { packer_ = p; packer_ctx_ = ctx; }
+
+    // readbacks (synchronous)
+    int read_x(std::vector<float2> &out);
+    int read_f32(const DBuf &b, std::vector<float> &out, size_t n);
+    int read_u32(const DBuf &b, std::vector<uint32_t> &out, size_t n);
+    int read_u64(const DBuf &b, std::vector<uint64_t> &out, size_t n);
+    int spikes_total(uint64_t &out);
+    int events_total(uint64_t &out);
+    int drain_raster();   // device raster window -> host vectors
+
+    // state
+    uint32_t n_nets = 0, n_total = 0, D = 2, words = 0, bits_rows = 0;
+    int ring_int = 0, pmode = 0, mmode = 0, packed = 1, fused = 0, use_graph = 0;
+    uint64_t t = 0;            // arena steps done
+    uint64_t n_syn = 0, n_in1 = 0;
+    std::vector<NetSlot> slots;
+    std::vector<NetRec> recs;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_run_ms = 0.0;
+    uint32_t stim_cols = 0, stim_cap = 0;
+    uint32_t n_watch = 0, xrec_depth = 4096;
+    bool has_ext = false;
+    uint64_t dev_bytes = 0;
+
+    // raster (host side accumulation)
+    uint32_t rast_steps = 0;
+    uint64_t rast_cap_ids = 0;
+    uint64_t rast_id0 = 0, rast_step0 = 0;      // origin of the device window
+    std::vector<uint32_t> h_rast_cnt;           // per step counts (drained)
+    std::vector<uint32_t> h_rast_ids;           // ids (ascending per step)
+    uint64_t h_rast_step0 = 0;                  // first step held on the host
+
+    // device buffers
+    DBuf d_t, d_nets, d_wave_net, d_x, d_x0, d_s, d_ring, d_ext, d_prow, d_ptab, d_p, d_mreg;
+    DBuf d_stim_ptr, d_stim_input, d_stim_w, d_stim_win;
+    DBuf d_in1_ptr, d_in1_src, d_in1_w, d_pbits;
+    DBuf d_spk, d_cnt, d_bits, d_pn, d_last, d_net_spikes;
+    DBuf d_rast_ids, d_rast_off, d_rast_ctl;
+    DBuf d_src_ptr, d_syn, d_delay16, d_w, d_skip1, d_totals;
+    DBuf d_watch, d_xrec;
+    float *h_stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    int stage_i = 0;
+    bool stage_used[2] = {false, false};
+
+    StepArgs args() const;
+    int launch_chunk(uint32_t k);
+    // direct launches with an event pair around every step kernel
+    int profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,
+                uint64_t &aux_n);
+    int capture_graph();
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+
+    StimPacker packer_ = nullptr;
+    void *packer_ctx_ = nullptr;
+    uint32_t P_ = 4;
+    bool graph_dirty = true;
+    uint32_t graph_steps = 64;
+    uint32_t scatter_blocks = 0;
+    bool timing_valid = false;
+    std::vector<uint32_t> h_src_ptr_;   // host copy of the arena CSR row starts
+
+  private:
+    int alloc(DBuf &b, size_t bytes);
+    template <class T> int upload(DBuf &b, const std::vector<T> &v);
+};
+
+}  // namespace nh

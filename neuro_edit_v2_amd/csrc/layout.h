@@ -1,0 +1,105 @@
+// layout.h -- device-memory layout shared by the host code and the kernels.
+//
+// One "arena" of neurons holds one network (plain path) or many networks
+// packed back to back (batched / served path).  Every network's neuron range
+// starts on a 64-neuron boundary, so a wave64 always covers neurons of one
+// network and its spike ballot is exactly one 64-bit word of the spike
+// bitmap.  Per-network scalars live in a NetRec table indexed per wave.
+#pragma once
+#include <stdint.h>
+
+namespace nh {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;          // 4 waves per workgroup
+constexpr int kMaxParams = 10;       // AdEx row width
+constexpr uint32_t kNoSpike = 0xFFFFFFFFu;
+
+// Per-network scalars (one row per packed network).  All fields are read
+// with scalar loads (the index is wave-uniform).
+struct alignas(64) NetRec {
+    uint32_t base;        // first arena neuron of the network
+    uint32_t n;           // neurons of the network
+    uint64_t seed;        // noise seed
+    int64_t t0;           // arena step at which the network's local step is 0
+    float pkick;          // rate*dt/1000 (Poisson-kick probability)
+    float kick;           // kick_mv/dt (0 when kick_mv == 0)
+    float sd;             // Gaussian current sd (0 = off)
+    float decay_s;        // exp(-dt/tau_syn)
+    float int_scale;      // int ring scale
+    uint32_t stim_base;   // first column of this network in the stimulus window
+    uint32_t active;      // 0 = slot empty (served form)
+    float dt;             // step, ms
+};
+static_assert(sizeof(NetRec) == 64, "NetRec is one cache line");
+
+// Parameter table row (type-indexed parameters: most networks have a few
+// distinct rows, e.g. RS and FS).
+struct alignas(16) PRow {
+    float p[kMaxParams];
+    uint32_t model;       // registry id: 0 izh, 1 adex
+    uint32_t pad;
+};
+static_assert(sizeof(PRow) == 48, "PRow size");
+
+// Kernel arguments of one step.  Pointers may be null when a feature is off.
+struct StepArgs {
+    uint32_t n_total;            // arena neurons (multiple of 64 for batches)
+    uint32_t D;                  // ring depth (max over packed networks)
+    const uint64_t *t_dev;       // arena step of chunk start (device)
+    uint32_t k;                  // step offset within the chunk
+    uint32_t single_net;         // 1: every wave uses nets[0]
+    const uint16_t *wave_net;    // per wave: index into nets
+    const NetRec *nets;
+    // state
+    float2 *x;                   // (v, u|w) per neuron
+    float *s;                    // decaying synaptic current
+    void *ring;                  // D x n_total (float or int32)
+    const float *ext;            // external current or null
+    // parameters
+    const uint8_t *prow;         // per-neuron row index (table mode)
+    const PRow *ptab;
+    const float *p;              // per-neuron rows (n_total x P) in row mode
+    const uint8_t *mreg;         // per-neuron registry model id (row mode)
+    uint32_t P;
+    // stimulus
+    const uint32_t *stim_ptr;    // n_total+1 or null
+    const uint32_t *stim_input;
+    const float *stim_w;
+    const float *stim_win;       // [chunk step][columns]
+    uint32_t stim_cols;
+    // delay-1 pull (fused mode): target-major in-lists of delay-1 synapses
+    const uint32_t *in1_ptr;     // n_total+1 or null
+    const uint32_t *in1_src;
+    const void *in1_w;           // float or int32
+    uint64_t *pbits;             // 2 x words: spike bits of the last two steps
+    // spike outputs
+    uint32_t *spk;               // spike lists (3 x n_total in fused mode, 2 x otherwise)
+    uint32_t *cnt;               // per-list counters
+    uint64_t *bits;              // record ring (rows x words) or null
+    uint32_t bits_rows;
+    uint32_t words;
+    uint32_t *pn_total;          // per-neuron lifetime spikes
+    uint32_t *last_spike;        // per-neuron last spike step (kNoSpike = never)
+    uint64_t *net_spikes;        // per-network spike totals or null
+    // raster stream
+    uint32_t *rast_ids;
+    uint32_t *rast_off;          // per step: ids before it in the window
+    uint64_t *rast_ctl;          // [0] id origin, [1] step origin, [2] overflow
+    uint64_t rast_cap_ids;
+    uint32_t rast_cap_steps;
+    // scatter
+    const uint32_t *src_ptr;     // n_total+1
+    const uint32_t *syn;         // packed (tgt | delay << 27) or tgt
+    const uint16_t *delay16;     // unpacked delays (null when packed)
+    const void *w;               // float or int32 per synapse
+    const uint32_t *skip1;       // per source: delay-1 synapses at the row head (fused mode)
+    uint64_t *totals;            // [1] events, [2+L] spikes before the step using list L
+    // watched neurons (record ring of state rows)
+    const uint32_t *watch;
+    uint32_t n_watch;
+    float2 *xrec;
+    uint32_t xrec_depth;
+};
+
+}  // namespace nh
