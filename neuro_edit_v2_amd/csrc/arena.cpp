@@ -163,8 +163,6 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     std::vector<int32_t> wq;
     if (ring_int) wq.resize(std::max<uint64_t>(total_syn, 1));
     else wf.resize(std::max<uint64_t>(total_syn, 1));
-    std::vector<uint32_t> skip1(n_total + 1, 0);
-    std::vector<uint32_t> in1_cnt(n_total + 1, 0);
     {
         uint64_t o = 0;
         std::vector<uint32_t> idx;
@@ -181,7 +179,6 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
                     if (d.delay[x] != d.delay[y]) return d.delay[x] < d.delay[y];
                     return d.tgt[x] < d.tgt[y];
                 });
-                uint32_t s1 = 0;
                 for (uint32_t j : idx) {
                     const uint32_t tg = b + d.tgt[j];
                     if (packed) syn[o] = tg | ((uint32_t)d.delay[j] << 27);
@@ -191,13 +188,8 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
                     }
                     if (ring_int) wq[o] = quantize(d.w[j], scale);
                     else wf[o] = d.w[j];
-                    if (d.delay[j] == 1) {
-                        ++s1;
-                        in1_cnt[tg + 1]++;
-                    }
                     ++o;
                 }
-                skip1[b + i] = s1;
             }
             // padding neurons of this network have empty rows
             for (uint32_t i = d.n; i < align64(d.n); ++i) src_ptr[b + i] = (uint32_t)o;
@@ -205,31 +197,8 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         src_ptr[n_total] = (uint32_t)o;
         if (o > 0xFFFFFFFFull) return set_error(NH_ERR_CAPACITY, "more than 2^32 synapses");
     }
-    // delay-1 in-lists (target-major), sources ascending
-    std::vector<uint32_t> in1_ptr(n_total + 1, 0);
-    for (uint32_t i = 0; i < n_total; ++i) in1_ptr[i + 1] = in1_ptr[i] + in1_cnt[i + 1];
-    n_in1 = in1_ptr[n_total];
-    std::vector<uint32_t> in1_src(std::max<uint64_t>(n_in1, 1));
-    std::vector<float> in1_wf;
-    std::vector<int32_t> in1_wq;
-    if (ring_int) in1_wq.resize(std::max<uint64_t>(n_in1, 1));
-    else in1_wf.resize(std::max<uint64_t>(n_in1, 1));
-    {
-        std::vector<uint32_t> fill(n_total, 0);
-        for (uint32_t s = 0; s < n_total; ++s) {
-            const uint32_t lo = src_ptr[s], n1 = skip1[s];
-            for (uint32_t j = lo; j < lo + n1; ++j) {
-                const uint32_t tg = packed ? (syn[j] & 0x07FFFFFFu) : syn[j];
-                const uint32_t o = in1_ptr[tg] + fill[tg]++;
-                in1_src[o] = s;
-                if (ring_int) in1_wq[o] = wq[j];
-                else in1_wf[o] = wf[j];
-            }
-        }
-    }
-    // fused shape when the pull is cheap next to the neuron update
-    if (tune.fused >= 0) fused = tune.fused;
-    else fused = (n_in1 <= (uint64_t)n_total / 2) ? 1 : 0;
+    // one kernel per step (own-spike push) unless the two-kernel shape is asked for
+    fused = tune.fused == 0 ? 0 : 1;
 
     // ---- parameters: type table when the arena has <= 256 distinct rows
     std::map<std::vector<uint32_t>, uint32_t> rows;
@@ -262,7 +231,10 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
             prow[slots[k].base + i] = (uint8_t)r;
         }
     }
-    pmode = table_ok ? 0 : 1;
+    // <= 4 rows ride in the kernel arguments, <= 256 in a global table,
+    // otherwise per-neuron rows
+    pmode = !table_ok ? 1 : (ptab.size() <= (size_t)kKargRows ? 0 : 2);
+    krows_ = ptab;
     std::vector<float> pflat;
     std::vector<uint8_t> mreg;
     if (pmode == 1) {
@@ -355,14 +327,6 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
             HIPCHK(hipEventCreateWithFlags(&stage_ev[i], hipEventDisableTiming));
         }
     }
-    if (fused) {
-        TRY(upload(d_in1_ptr, in1_ptr));
-        TRY(upload(d_in1_src, in1_src));
-        if (ring_int) TRY(upload(d_in1_w, in1_wq));
-        else TRY(upload(d_in1_w, in1_wf));
-        TRY(upload(d_skip1, skip1));
-    }
-    TRY(alloc(d_pbits, (size_t)2 * std::max<uint32_t>(words, 1) * 8));
     TRY(alloc(d_spk, (size_t)3 * std::max<uint32_t>(n_total, 1) * 4));
     TRY(alloc(d_cnt, 4 * 4));
     bits_rows = 0;
@@ -392,7 +356,8 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     HIPCHK(hipDeviceSynchronize());
 
     use_graph = tune.use_graph >= 0 ? tune.use_graph : 1;
-    graph_steps = tune.graph_steps ? tune.graph_steps : 64;
+    graph_steps_req = tune.graph_steps ? tune.graph_steps : 64;
+    graph_steps = 0;
     scatter_blocks = tune.scatter_blocks;
     return NH_OK;
 }
@@ -420,13 +385,17 @@ StepArgs Arena::args() const {
     a.D = D;
     a.t_dev = d_t.as<uint64_t>();
     a.k = 0;
+    a.slot = 0;
     a.single_net = n_nets == 1 ? 1 : 0;
     a.wave_net = d_wave_net.as<uint16_t>();
     a.nets = d_nets.as<NetRec>();
+    a.net0 = recs[0];
     a.x = d_x.as<float2>();
     a.s = d_s.as<float>();
     a.ring = d_ring.p;
     a.ext = has_ext ? d_ext.as<float>() : nullptr;
+    a.n_krows = pmode == 0 ? (uint32_t)krows_.size() : 0;
+    for (size_t r = 0; r < krows_.size() && r < (size_t)kKargRows; ++r) a.krow[r] = krows_[r];
     a.prow = d_prow.as<uint8_t>();
     a.ptab = d_ptab.as<PRow>();
     a.p = d_p.as<float>();
@@ -437,10 +406,6 @@ StepArgs Arena::args() const {
     a.stim_w = d_stim_w.as<float>();
     a.stim_win = d_stim_win.as<float>();
     a.stim_cols = stim_cols;
-    a.in1_ptr = d_in1_ptr.as<uint32_t>();
-    a.in1_src = d_in1_src.as<uint32_t>();
-    a.in1_w = d_in1_w.p;
-    a.pbits = fused ? d_pbits.as<uint64_t>() : nullptr;
     a.spk = d_spk.as<uint32_t>();
     a.cnt = d_cnt.as<uint32_t>();
     a.bits = bits_rows ? d_bits.as<uint64_t>() : nullptr;
@@ -458,7 +423,6 @@ StepArgs Arena::args() const {
     a.syn = d_syn.as<uint32_t>();
     a.delay16 = d_delay16.as<uint16_t>();
     a.w = d_w.p;
-    a.skip1 = d_skip1.as<uint32_t>();
     a.totals = d_totals.as<uint64_t>();
     a.watch = d_watch.as<uint32_t>();
     a.n_watch = n_watch;
@@ -476,12 +440,18 @@ uint32_t Arena::scat_blocks() const {
     return std::min<uint32_t>(std::max<uint32_t>(neuron_blocks(), 1), 2048);
 }
 
-int Arena::launch_chunk(uint32_t k) {
+bool Arena::list_on() const { return !fused || rast_steps != 0; }
+
+// Enqueue k steps starting at arena step t0 (t0 known to the host; the
+// kernels read the absolute step from d_t, which holds t0 at this point).
+int Arena::launch_chunk(uint64_t t0, uint32_t k) {
     StepArgs a = args();
     const uint32_t nb = neuron_blocks(), sb = scat_blocks();
+    const bool list = list_on();
     for (uint32_t j = 0; j < k; ++j) {
         a.k = j;
-        launch_neuron(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
+        a.slot = (uint32_t)((t0 + j) % D);
+        launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, list, stream);
         if (!fused) launch_scatter(a, sb, ring_int, packed != 0, stream);
         if (n_watch) launch_watch(a, stream);
     }
@@ -503,6 +473,7 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     int rc = NH_OK;
     const uint32_t nb = neuron_blocks(), sb = scat_blocks();
+    const bool list = list_on();
     while (n_steps && rc == NH_OK) {
         uint64_t k = std::min<uint64_t>(n_steps, std::min<uint32_t>(kPool, stim_cap));
         if (rast_steps) {
@@ -517,8 +488,9 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
         StepArgs a = args();
         for (uint32_t j = 0; j < k; ++j) {
             a.k = j;
+            a.slot = (uint32_t)((t + j) % D);
             (void)hipEventRecord(ev[4 * j], stream);
-            launch_neuron(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
+            launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, list, stream);
             (void)hipEventRecord(ev[4 * j + 1], stream);
             if (!fused) {
                 (void)hipEventRecord(ev[4 * j + 2], stream);
@@ -551,6 +523,9 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
     return rc;
 }
 
+// A captured chunk covers graph_steps steps, a multiple of the ring depth,
+// and is only replayed from arena steps t = 0 (mod D): the ring slot of
+// every step is then fixed per graph node.
 int Arena::capture_graph() {
     if (graph_exec) {
         (void)hipGraphExecDestroy(graph_exec);
@@ -561,7 +536,7 @@ int Arena::capture_graph() {
         graph = nullptr;
     }
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_chunk(graph_steps);
+    int rc = launch_chunk(0, graph_steps);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(stream, &g);
     if (rc) return rc;
@@ -590,11 +565,23 @@ int Arena::run(uint64_t n_steps) {
         last_run_ms = 0.0;
         return NH_OK;
     }
+    if (use_graph) {
+        // graph chunk: a multiple of D near the requested length
+        const uint32_t want = std::max<uint32_t>(graph_steps_req, 1);
+        const uint32_t g = std::max<uint32_t>(1, (want + D / 2) / D) * D;
+        if (g != graph_steps) {
+            graph_steps = g;
+            graph_dirty = true;
+        }
+        if (graph_steps > stim_cap) use_graph = 0;
+    }
     if (use_graph && graph_dirty) TRY(capture_graph());
     HIPCHK(hipEventRecord(ev0, stream));
     while (n_steps) {
-        uint64_t k = std::min<uint64_t>(n_steps, use_graph ? graph_steps : stim_cap);
-        k = std::min<uint64_t>(k, stim_cap);
+        uint64_t k = std::min<uint64_t>(n_steps, stim_cap);
+        bool graph_ok = use_graph != 0 && t % D == 0;
+        if (use_graph && !graph_ok) k = std::min<uint64_t>(k, D - t % D);  // realign
+        if (graph_ok) k = std::min<uint64_t>(k, graph_steps);
         if (rast_steps) {
             const uint64_t room = rast_steps - (t - rast_step0);
             if (room == 0) {
@@ -604,8 +591,8 @@ int Arena::run(uint64_t n_steps) {
             k = std::min<uint64_t>(k, room);
         }
         TRY(upload_stim(t, (uint32_t)k));
-        if (use_graph && k == graph_steps) HIPCHK(hipGraphLaunch(graph_exec, stream));
-        else TRY(launch_chunk((uint32_t)k));
+        if (graph_ok && k == graph_steps) HIPCHK(hipGraphLaunch(graph_exec, stream));
+        else TRY(launch_chunk(t, (uint32_t)k));
         t += k;
         n_steps -= k;
     }
@@ -629,7 +616,6 @@ int Arena::reset() {
     HIPCHK(hipMemcpyAsync(d_x.p, d_x0.p, d_x0.bytes, hipMemcpyDeviceToDevice, stream));
     HIPCHK(hipMemsetAsync(d_s.p, 0, d_s.bytes, stream));
     HIPCHK(hipMemsetAsync(d_ring.p, 0, d_ring.bytes, stream));
-    HIPCHK(hipMemsetAsync(d_pbits.p, 0, d_pbits.bytes, stream));
     HIPCHK(hipMemsetAsync(d_cnt.p, 0, d_cnt.bytes, stream));
     HIPCHK(hipMemsetAsync(d_totals.p, 0, d_totals.bytes, stream));
     if (d_bits.p) HIPCHK(hipMemsetAsync(d_bits.p, 0, d_bits.bytes, stream));
@@ -679,11 +665,8 @@ int Arena::spikes_total(uint64_t &out) {
     out = 0;
     if (t == 0 || n_total == 0) return NH_OK;
     std::vector<uint64_t> tot;
-    std::vector<uint32_t> cnt;
     TRY(read_u64(d_totals, tot, 8));
-    TRY(read_u32(d_cnt, cnt, 4));
-    const uint32_t L = (uint32_t)((t - 1) % 3);
-    out = tot[2 + L] + cnt[L];
+    out = tot[0];
     return NH_OK;
 }
 
@@ -693,19 +676,6 @@ int Arena::events_total(uint64_t &out) {
     std::vector<uint64_t> tot;
     TRY(read_u64(d_totals, tot, 8));
     out = tot[1];
-    if (fused) {
-        // the last step's spikes are pushed by the next step's launch
-        std::vector<uint32_t> cnt;
-        TRY(read_u32(d_cnt, cnt, 4));
-        const uint32_t L = (uint32_t)((t - 1) % 3);
-        const uint32_t c = cnt[L];
-        if (c) {
-            std::vector<uint32_t> ids(c);
-            HIPCHK(hipMemcpy(ids.data(), d_spk.as<uint32_t>() + (size_t)L * n_total, (size_t)c * 4,
-                             hipMemcpyDeviceToHost));
-            for (uint32_t i : ids) out += h_src_ptr_[i + 1] - h_src_ptr_[i];
-        }
-    }
     return NH_OK;
 }
 

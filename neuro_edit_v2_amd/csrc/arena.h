@@ -13,8 +13,8 @@
 
 namespace nh {
 
-void launch_neuron(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool fused,
-                   bool packed, hipStream_t st);
+void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool push,
+                 bool packed, bool list, hipStream_t st);
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st);
 void launch_watch(const StepArgs &a, hipStream_t st);
 void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
@@ -96,7 +96,7 @@ class Arena {
     uint32_t n_nets = 0, n_total = 0, D = 2, words = 0, bits_rows = 0;
     int ring_int = 0, pmode = 0, mmode = 0, packed = 1, fused = 0, use_graph = 0;
     uint64_t t = 0;            // arena steps done
-    uint64_t n_syn = 0, n_in1 = 0;
+    uint64_t n_syn = 0;
     std::vector<NetSlot> slots;
     std::vector<NetRec> recs;
     hipStream_t stream = nullptr;
@@ -118,10 +118,9 @@ class Arena {
     // device buffers
     DBuf d_t, d_nets, d_wave_net, d_x, d_x0, d_s, d_ring, d_ext, d_prow, d_ptab, d_p, d_mreg;
     DBuf d_stim_ptr, d_stim_input, d_stim_w, d_stim_win;
-    DBuf d_in1_ptr, d_in1_src, d_in1_w, d_pbits;
     DBuf d_spk, d_cnt, d_bits, d_pn, d_last, d_net_spikes;
     DBuf d_rast_ids, d_rast_off, d_rast_ctl;
-    DBuf d_src_ptr, d_syn, d_delay16, d_w, d_skip1, d_totals;
+    DBuf d_src_ptr, d_syn, d_delay16, d_w, d_totals;
     DBuf d_watch, d_xrec;
     float *h_stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
@@ -129,7 +128,8 @@ class Arena {
     bool stage_used[2] = {false, false};
 
     StepArgs args() const;
-    int launch_chunk(uint32_t k);
+    int launch_chunk(uint64_t t0, uint32_t k);
+    bool list_on() const;
     // direct launches with an event pair around every step kernel
     int profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,
                 uint64_t &aux_n);
@@ -141,8 +141,9 @@ class Arena {
     void *packer_ctx_ = nullptr;
     uint32_t P_ = 4;
     bool graph_dirty = true;
-    uint32_t graph_steps = 64;
+    uint32_t graph_steps = 0, graph_steps_req = 64;
     uint32_t scatter_blocks = 0;
+    std::vector<PRow> krows_;
     bool timing_valid = false;
     std::vector<uint32_t> h_src_ptr_;   // host copy of the arena CSR row starts
 

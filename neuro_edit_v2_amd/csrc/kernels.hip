@@ -1,22 +1,25 @@
 // kernels.hip -- the per-step hot path for gfx950 (wave64, CDNA4).
 //
-// Two execution shapes, both parity-equal to the CPU step order of
-// sim/src/cpu.rs:192-306:
+// Step t of the reference CPU backend (sim/src/cpu.rs:192-306) is: read and
+// clear ring slot t, integrate every neuron, then scatter the spikes of
+// step t into slots t+delay (delay >= 1).  The scatter of step t never
+// touches slot t, and the update of step t reads no slot but t, so one
+// kernel per step suffices:
 //
-//  * two-phase: k_neuron (ring read/clear, decay, noise, model step,
-//    wave64 ballot compaction into the step's spike list) then k_scatter
-//    (one wave per spike, lanes over the delay-sorted CSR row, hardware
-//    atomics into the delay ring);
+//  * k_step<PUSH=true> ("fused" shape): each wave updates its 64 neurons,
+//    ballots its spikes (the wave64 mask is one word of the spike bitmap),
+//    then pushes the out-synapse rows of its own spiking neurons into the
+//    ring with hardware atomics (lanes stride the delay-sorted row).  No
+//    cross-wave dependency exists inside a step, so there is no grid
+//    barrier; the kernel boundary orders step t's pushes before step t+1's
+//    reads.
 //
-//  * fused: ONE kernel per step.  Synapses with delay >= 2 of the previous
-//    step's spikes are pushed (they land in slots this step never reads),
-//    delay-1 contributions are pulled per target from a target-major list
-//    and the previous step's spike bitmap.  No grid-wide barrier is needed
-//    inside a step.
+//  * k_step<PUSH=false> + k_scatter ("two-kernel" shape): the update
+//    compacts the spikes of the step into a list; a second kernel scatters
+//    one spike per wave.  Kept as the reference shape and for comparison.
 //
-// Spike lists rotate over three buffers (step t uses list t%3); counter and
-// cumulative-total bookkeeping is done by the first thread of the grid
-// without races (see k_neuron).
+// The int ring (sim/src/kernels.rs:57-66) sums integers, which commute, so
+// both shapes give the CPU raster bit for bit whatever the atomic order.
 #include <hip/hip_runtime.h>
 #include "device.h"
 #include "layout.h"
@@ -45,8 +48,8 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-// Push the out-synapses of one spike into the ring: lanes stride the row
-// (delay-sorted, so a contiguous run of lanes hits one ring row).
+// Push one spike's out-synapse row into the ring; lanes stride the row
+// (delay-sorted, so a run of lanes hits one ring row with ascending targets).
 template <int RING, bool PACKED>
 __device__ __forceinline__ void push_row(const StepArgs &a, uint32_t lo, uint32_t hi,
                                          uint32_t slot0, uint32_t lane) {
@@ -69,80 +72,65 @@ __device__ __forceinline__ void push_row(const StepArgs &a, uint32_t lo, uint32_
     }
 }
 
-// ---------------------------------------------------------------- neuron
-template <int RING, int PMODE, int MMODE, bool FUSED, bool PACKED>
-__global__ __launch_bounds__(kBlock) void k_neuron(StepArgs a) {
+// ---------------------------------------------------------------- step
+// PMODE: 0 parameter rows in the kernel arguments (<= 4 rows), 1 per-neuron
+// rows, 2 global row table.  MMODE: 0 izh, 1 adex, 2 mixed.
+// LIST: keep the per-step spike list (two-kernel shape or raster on).
+template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LIST>
+__global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
     using RT = typename Ring<RING>::T;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & (kWave - 1);
-    const uint32_t wib = uniform(tid >> 6);
-    const uint32_t wave = blockIdx.x * (kBlock / kWave) + wib;
-    const uint32_t i = wave * kWave + lane;
-    const uint64_t t = *a.t_dev + a.k;
-    const uint32_t D = a.D;
-    const uint32_t slot = (uint32_t)(t % D);
-    const uint32_t L = (uint32_t)(t % 3), Lp = (L + 2) % 3, Ln = (L + 1) % 3;
-    // Cumulative spikes before step t = before step t-1 + count of t-1;
-    // both are final (written by earlier launches), so every wave may read
-    // them.  fill[] lives in totals[2..4].
-    const uint64_t base_t = a.totals[2 + Lp] + a.cnt[Lp];
-    if (blockIdx.x == 0 && tid == 0) {
-        a.totals[2 + L] = base_t;
-        a.cnt[Ln] = 0;
-        if (a.rast_off) {
-            const uint64_t rs = t - a.rast_ctl[1];
-            if (rs < a.rast_cap_steps) a.rast_off[rs] = (uint32_t)(base_t - a.rast_ctl[0]);
-        }
-    }
-
-    // Fused shape: push the previous step's delay>=2 synapses first.
-    if (FUSED && t > 0) {
-        const uint32_t cprev = a.cnt[Lp];
-        const uint32_t nwaves = gridDim.x * (kBlock / kWave);
-        const uint32_t slotp = (slot + D - 1) % D;  // (t-1) % D
-        uint64_t ev = 0;
-        for (uint32_t q = wave; q < cprev; q += nwaves) {
-            const uint32_t src = uniform(a.spk[(size_t)Lp * a.n_total + q]);
-            const uint32_t lo = a.src_ptr[src], hi = a.src_ptr[src + 1];
-            push_row<RING, PACKED>(a, lo + a.skip1[src], hi, slotp, lane);
-            ev += hi - lo;
-        }
-        if (lane == 0 && ev) atomicAdd((unsigned long long *)&a.totals[1], (unsigned long long)ev);
-    }
-
-    // waves past the arena (last block) only helped with the push above
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wave = blockIdx.x * (kBlock / kWave) + uniform(threadIdx.x >> 6);
     if (wave >= a.words) return;
+    const uint32_t i = wave * kWave + lane;
+    const uint32_t slot = a.slot;
     const uint32_t net = a.single_net ? 0u : uniform((uint32_t)a.wave_net[wave]);
-    const NetRec &nr = a.nets[net];
+    const NetRec &nr = a.single_net ? a.net0 : a.nets[net];
+
+    // ---- first batch of independent loads
+    RT *ring = (RT *)a.ring;
+    const size_t ri = (size_t)slot * a.n_total + i;
+    const RT rv = ring[ri];
+    const float s_old = a.s[i];
+    float2 xs = a.x[i];
+    const float ext = a.ext ? a.ext[i] : 0.0f;
+    uint32_t row_lo = 0, row_hi = 0;
+    if (PUSH) {
+        row_lo = a.src_ptr[i];
+        row_hi = a.src_ptr[i + 1];
+    }
+    uint32_t prow = 0;
+    if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
+    const uint64_t t = *a.t_dev + a.k;
+
     const uint32_t local = i - nr.base;
     const bool act = local < nr.n && nr.active;
     bool spiked = false;
     if (act) {
-        RT *ring = (RT *)a.ring;
-        const size_t ri = (size_t)slot * a.n_total + i;
-        RT rv = ring[ri];
         ring[ri] = (RT)0;
-        if (FUSED) {
-            // delay-1 arrivals from the previous step, pulled
-            const uint64_t *pb = a.pbits + (size_t)((t + 1) & 1) * a.words;
-            const RT *w1 = (const RT *)a.in1_w;
-            for (uint32_t j = a.in1_ptr[i]; j < a.in1_ptr[i + 1]; ++j) {
-                const uint32_t src = a.in1_src[j];
-                if ((pb[src >> 6] >> (src & 63)) & 1ull) rv += w1[j];
-            }
-        }
         const float arr = Ring<RING>::get(rv, nr.int_scale);
-        const float sv = a.s[i] + arr;
-        float I = sv + (a.ext ? a.ext[i] : 0.0f);
+        const float sv = s_old + arr;
+        float I = sv + ext;
         if (a.stim_ptr) {
             const float *row = a.stim_win + (size_t)a.k * a.stim_cols + nr.stim_base;
             for (uint32_t j = a.stim_ptr[i]; j < a.stim_ptr[i + 1]; ++j)
                 I += a.stim_w[j] * row[a.stim_input[j]];
         }
         float pr[kMaxParams];
-        uint32_t model;
+        uint32_t model = (uint32_t)MMODE;
         if (PMODE == 0) {
-            const PRow &r = a.ptab[a.prow[i]];
+#pragma unroll
+            for (int q = 0; q < kMaxParams; ++q) pr[q] = a.krow[0].p[q];
+            model = a.krow[0].model;
+#pragma unroll
+            for (int r = 1; r < kKargRows; ++r) {
+                const bool sel = prow == (uint32_t)r;
+#pragma unroll
+                for (int q = 0; q < kMaxParams; ++q) pr[q] = sel ? a.krow[r].p[q] : pr[q];
+                model = sel ? a.krow[r].model : model;
+            }
+        } else if (PMODE == 2) {
+            const PRow &r = a.ptab[prow];
 #pragma unroll
             for (int q = 0; q < kMaxParams; ++q) pr[q] = r.p[q];
             model = r.model;
@@ -157,16 +145,15 @@ __global__ __launch_bounds__(kBlock) void k_neuron(StepArgs a) {
             }
         }
         const bool is_adex = MMODE == 1 || (MMODE == 2 && model == 1);
+        // sim/src/cpu.rs:240-245: kick * model gain (C for AdEx, adex.rs:189-191)
         const float gain = is_adex ? pr[0] : 1.0f;
         const uint64_t tl = t - (uint64_t)nr.t0;
         if (nr.pkick > 0.0f && kick_fires(local, tl, nr.seed, nr.pkick)) I += nr.kick * gain;
         if (nr.sd != 0.0f) I += (nr.sd * gauss_draw(local, tl, nr.seed)) * gain;
-        float2 xs = a.x[i];
-        const float dt = nr.dt;
         if (is_adex)
-            spiked = adex_step(pr, xs, I, dt);
+            spiked = adex_step(pr, xs, I, nr.dt);
         else
-            spiked = izh_step(pr[0], pr[1], pr[2], pr[3], xs, I, dt);
+            spiked = izh_step(pr[0], pr[1], pr[2], pr[3], xs, I, nr.dt);
         a.x[i] = xs;
         a.s[i] = sv * nr.decay_s;
         if (spiked) {
@@ -175,14 +162,29 @@ __global__ __launch_bounds__(kBlock) void k_neuron(StepArgs a) {
         }
     }
 
-    // wave64 ballot: the mask IS this wave's word of the spike bitmap
+    // ---- wave64 ballot: the mask IS this wave's word of the spike bitmap
     const uint64_t mask = __ballot(spiked);
-    if (lane == 0) {
-        if (a.bits) a.bits[(size_t)(t % a.bits_rows) * a.words + wave] = mask;
-        if (a.pbits) a.pbits[(size_t)(t & 1) * a.words + wave] = mask;
+    if (a.bits && lane == 0) a.bits[(size_t)(t % a.bits_rows) * a.words + wave] = mask;
+    const uint32_t L = (uint32_t)(t % 3);
+    if (LIST && blockIdx.x == 0 && threadIdx.x == 0) {
+        // spike-list bookkeeping of step t (every value read here is final:
+        // written by earlier launches)
+        const uint32_t Lp = (L + 2) % 3, Ln = (L + 1) % 3;
+        const uint64_t base_t = a.totals[2 + Lp] + a.cnt[Lp];
+        a.totals[2 + L] = base_t;
+        a.cnt[Ln] = 0;
+        if (a.rast_off) {
+            const uint64_t rs = t - a.rast_ctl[1];
+            if (rs < a.rast_cap_steps) a.rast_off[rs] = (uint32_t)(base_t - a.rast_ctl[0]);
+        }
     }
+    if (!mask) return;
     const uint32_t nsp = (uint32_t)__popcll(mask);
-    if (nsp) {
+    if (lane == 0) {
+        atomicAdd((unsigned long long *)&a.totals[0], (unsigned long long)nsp);
+        if (a.net_spikes) atomicAdd((unsigned long long *)&a.net_spikes[net], (unsigned long long)nsp);
+    }
+    if (LIST) {
         uint32_t off = 0;
         if (lane == 0) off = atomicAdd(&a.cnt[L], nsp);
         off = uniform(off);
@@ -190,6 +192,8 @@ __global__ __launch_bounds__(kBlock) void k_neuron(StepArgs a) {
             const uint32_t pos = off + wave_rank(mask);
             a.spk[(size_t)L * a.n_total + pos] = i;
             if (a.rast_ids) {
+                const uint32_t Lp = (L + 2) % 3;
+                const uint64_t base_t = a.totals[2 + Lp] + a.cnt[Lp];
                 const uint64_t rp = base_t + pos - a.rast_ctl[0];
                 if (rp < a.rast_cap_ids)
                     a.rast_ids[rp] = i;
@@ -197,27 +201,37 @@ __global__ __launch_bounds__(kBlock) void k_neuron(StepArgs a) {
                     a.rast_ctl[2] = 1;
             }
         }
-        if (lane == 0 && a.net_spikes)
-            atomicAdd((unsigned long long *)&a.net_spikes[net], (unsigned long long)nsp);
+    }
+    if (PUSH) {
+        // push the rows of this wave's spikes, one spike at a time, all lanes
+        uint64_t m = mask, ev = 0;
+        while (m) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t lo = __builtin_amdgcn_readlane(row_lo, b);
+            const uint32_t hi = __builtin_amdgcn_readlane(row_hi, b);
+            push_row<RING, PACKED>(a, lo, hi, slot, lane);
+            ev += hi - lo;
+        }
+        if (lane == 0 && ev) atomicAdd((unsigned long long *)&a.totals[1], (unsigned long long)ev);
     }
 }
 
 // ---------------------------------------------------------------- scatter
-// Two-phase shape: one wave per spike of step t, all synapses of the row.
+// Two-kernel shape: one wave per spike of step t, all synapses of the row.
 template <int RING, bool PACKED>
 __global__ __launch_bounds__(kBlock) void k_scatter(StepArgs a) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = blockIdx.x * (kBlock / kWave) + uniform(threadIdx.x >> 6);
     const uint64_t t = *a.t_dev + a.k;
     const uint32_t L = (uint32_t)(t % 3);
-    const uint32_t slot0 = (uint32_t)(t % a.D);
     const uint32_t c = a.cnt[L];
     const uint32_t nwaves = gridDim.x * (kBlock / kWave);
     uint64_t ev = 0;
     for (uint32_t q = wave; q < c; q += nwaves) {
         const uint32_t src = uniform(a.spk[(size_t)L * a.n_total + q]);
         const uint32_t lo = a.src_ptr[src], hi = a.src_ptr[src + 1];
-        push_row<RING, PACKED>(a, lo, hi, slot0, lane);
+        push_row<RING, PACKED>(a, lo, hi, a.slot, lane);
         ev += hi - lo;
     }
     if (lane == 0 && ev) atomicAdd((unsigned long long *)&a.totals[1], (unsigned long long)ev);
@@ -236,44 +250,50 @@ __global__ void k_watch(StepArgs a) {
 __global__ void k_advance(uint64_t *t_dev, uint32_t k) { *t_dev += k; }
 
 // ------------------------------------------------------------ dispatch
-using KFn = void (*)(StepArgs);
-
-template <int RING, int PMODE, int MMODE, bool FUSED, bool PACKED>
-static void launch_neuron_t(const StepArgs &a, uint32_t blocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_neuron<RING, PMODE, MMODE, FUSED, PACKED>), dim3(blocks), dim3(kBlock), 0,
-                       st, a);
+template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LIST>
+static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LIST>), dim3(blocks), dim3(kBlock),
+                       0, st, a);
 }
 
 template <int RING, int PMODE, int MMODE>
-static void launch_neuron_f(const StepArgs &a, uint32_t blocks, bool fused, bool packed,
-                            hipStream_t st) {
-    if (fused) {
-        if (packed) launch_neuron_t<RING, PMODE, MMODE, true, true>(a, blocks, st);
-        else launch_neuron_t<RING, PMODE, MMODE, true, false>(a, blocks, st);
+static void launch_m(const StepArgs &a, uint32_t blocks, bool push, bool packed, bool list,
+                     hipStream_t st) {
+    if (!push) {
+        go<RING, PMODE, MMODE, false, false, true>(a, blocks, st);
+    } else if (packed) {
+        if (list) go<RING, PMODE, MMODE, true, true, true>(a, blocks, st);
+        else go<RING, PMODE, MMODE, true, true, false>(a, blocks, st);
     } else {
-        launch_neuron_t<RING, PMODE, MMODE, false, false>(a, blocks, st);
+        if (list) go<RING, PMODE, MMODE, true, false, true>(a, blocks, st);
+        else go<RING, PMODE, MMODE, true, false, false>(a, blocks, st);
     }
 }
 
 template <int RING, int PMODE>
-static void launch_neuron_m(const StepArgs &a, uint32_t blocks, int mmode, bool fused, bool packed,
-                            hipStream_t st) {
+static void launch_p(const StepArgs &a, uint32_t blocks, int mmode, bool push, bool packed, bool list,
+                     hipStream_t st) {
     switch (mmode) {
-    case 0: launch_neuron_f<RING, PMODE, 0>(a, blocks, fused, packed, st); break;
-    case 1: launch_neuron_f<RING, PMODE, 1>(a, blocks, fused, packed, st); break;
-    default: launch_neuron_f<RING, PMODE, 2>(a, blocks, fused, packed, st); break;
+    case 0: launch_m<RING, PMODE, 0>(a, blocks, push, packed, list, st); break;
+    case 1: launch_m<RING, PMODE, 1>(a, blocks, push, packed, list, st); break;
+    default: launch_m<RING, PMODE, 2>(a, blocks, push, packed, list, st); break;
     }
 }
 
-void launch_neuron(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool fused,
-                   bool packed, hipStream_t st) {
-    if (ring == 0) {
-        if (pmode == 0) launch_neuron_m<0, 0>(a, blocks, mmode, fused, packed, st);
-        else launch_neuron_m<0, 1>(a, blocks, mmode, fused, packed, st);
-    } else {
-        if (pmode == 0) launch_neuron_m<1, 0>(a, blocks, mmode, fused, packed, st);
-        else launch_neuron_m<1, 1>(a, blocks, mmode, fused, packed, st);
+template <int RING>
+static void launch_r(const StepArgs &a, uint32_t blocks, int pmode, int mmode, bool push, bool packed,
+                     bool list, hipStream_t st) {
+    switch (pmode) {
+    case 0: launch_p<RING, 0>(a, blocks, mmode, push, packed, list, st); break;
+    case 1: launch_p<RING, 1>(a, blocks, mmode, push, packed, list, st); break;
+    default: launch_p<RING, 2>(a, blocks, mmode, push, packed, list, st); break;
     }
+}
+
+void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool push,
+                 bool packed, bool list, hipStream_t st) {
+    if (ring == 0) launch_r<0>(a, blocks, pmode, mmode, push, packed, list, st);
+    else launch_r<1>(a, blocks, pmode, mmode, push, packed, list, st);
 }
 
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st) {

@@ -42,23 +42,29 @@ struct alignas(16) PRow {
 };
 static_assert(sizeof(PRow) == 48, "PRow size");
 
+constexpr int kKargRows = 4;        // parameter rows carried in the kernel arguments
+
 // Kernel arguments of one step.  Pointers may be null when a feature is off.
 struct StepArgs {
-    uint32_t n_total;            // arena neurons (multiple of 64 for batches)
+    uint32_t n_total;            // arena neurons (multiple of 64)
     uint32_t D;                  // ring depth (max over packed networks)
     const uint64_t *t_dev;       // arena step of chunk start (device)
     uint32_t k;                  // step offset within the chunk
-    uint32_t single_net;         // 1: every wave uses nets[0]
+    uint32_t slot;               // (t_dev + k) % D, fixed by the launcher
+    uint32_t single_net;         // 1: every wave uses net0 (no table read)
     const uint16_t *wave_net;    // per wave: index into nets
     const NetRec *nets;
+    NetRec net0;                 // the network of a single-network arena
     // state
     float2 *x;                   // (v, u|w) per neuron
     float *s;                    // decaying synaptic current
     void *ring;                  // D x n_total (float or int32)
     const float *ext;            // external current or null
     // parameters
-    const uint8_t *prow;         // per-neuron row index (table mode)
-    const PRow *ptab;
+    uint32_t n_krows;            // rows in krow (1..kKargRows) in kernel-argument mode
+    PRow krow[kKargRows];        // parameter rows carried in the arguments
+    const uint8_t *prow;         // per-neuron row index (table modes)
+    const PRow *ptab;            // global table (<= 256 rows)
     const float *p;              // per-neuron rows (n_total x P) in row mode
     const uint8_t *mreg;         // per-neuron registry model id (row mode)
     uint32_t P;
@@ -68,13 +74,8 @@ struct StepArgs {
     const float *stim_w;
     const float *stim_win;       // [chunk step][columns]
     uint32_t stim_cols;
-    // delay-1 pull (fused mode): target-major in-lists of delay-1 synapses
-    const uint32_t *in1_ptr;     // n_total+1 or null
-    const uint32_t *in1_src;
-    const void *in1_w;           // float or int32
-    uint64_t *pbits;             // 2 x words: spike bits of the last two steps
     // spike outputs
-    uint32_t *spk;               // spike lists (3 x n_total in fused mode, 2 x otherwise)
+    uint32_t *spk;               // per-step spike lists (3 x n_total)
     uint32_t *cnt;               // per-list counters
     uint64_t *bits;              // record ring (rows x words) or null
     uint32_t bits_rows;
@@ -88,13 +89,12 @@ struct StepArgs {
     uint64_t *rast_ctl;          // [0] id origin, [1] step origin, [2] overflow
     uint64_t rast_cap_ids;
     uint32_t rast_cap_steps;
-    // scatter
+    // synapses (CSR by source, rows sorted by delay then target)
     const uint32_t *src_ptr;     // n_total+1
     const uint32_t *syn;         // packed (tgt | delay << 27) or tgt
     const uint16_t *delay16;     // unpacked delays (null when packed)
     const void *w;               // float or int32 per synapse
-    const uint32_t *skip1;       // per source: delay-1 synapses at the row head (fused mode)
-    uint64_t *totals;            // [1] events, [2+L] spikes before the step using list L
+    uint64_t *totals;            // [0] spikes, [1] events, [2+L] list fill before step of list L
     // watched neurons (record ring of state rows)
     const uint32_t *watch;
     uint32_t n_watch;
