@@ -198,6 +198,17 @@ int nh_net_grid(uint32_t ex, uint32_t ey, uint32_t ez, double scale, double lamb
                 double inh_frac, float w_exc, float w_inh, int model, uint32_t n_inputs,
                 uint32_t per_input, float w_in, uint64_t seed, float dt_ms, nh_net **out);
 
+/* RngStream::window -- sim/src/script.rs:2849-2894: `steps` x `n_src`
+ * Bernoulli(p) bits from a seeded Rng; with event != 0 each listed pair
+ * (pairs[2k], pairs[2k+1]) shares one Bernoulli(p) bit with probability
+ * `strength` per step.  The generator state carries across calls on the
+ * same handle (nh_rng_stream_new/free). */
+typedef struct nh_rng_stream nh_rng_stream;
+int nh_rng_stream_new(uint64_t seed, uint32_t n_src, float p, nh_rng_stream **out);
+void nh_rng_stream_free(nh_rng_stream *s);
+int nh_rng_stream_window(nh_rng_stream *s, uint32_t steps, int event, const uint32_t *pairs,
+                         uint32_t n_pairs, float strength, float *out);
+
 /* ---------------- batched form ----------------
  * Many independent networks packed by offset tables into one launch
  * sequence; each network keeps its own config (dt must match), seed, t

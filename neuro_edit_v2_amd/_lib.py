@@ -105,6 +105,9 @@ def lib():
         "nh_net_cube": (i32, [u32, f64, u64, f32, P(vp)]),
         "nh_net_grid": (i32, [u32, u32, u32, f64, f64, f64, f32, f32, C.c_int, u32, u32, f32,
                               u64, f32, P(vp)]),
+        "nh_rng_stream_new": (i32, [u64, u32, f32, P(vp)]),
+        "nh_rng_stream_free": (None, [vp]),
+        "nh_rng_stream_window": (i32, [vp, u32, C.c_int, vp, u32, f32, vp]),
         "nh_batch_create": (i32, [u32, vp, vp, P(vp)]),
         "nh_batch_destroy": (None, [vp]),
         "nh_batch_run_steps": (i32, [vp, u64]),

@@ -419,7 +419,15 @@ StepArgs Arena::args() const {
     a.rast_ctl = d_rast_ctl.as<uint64_t>();
     a.rast_cap_ids = rast_cap_ids;
     a.rast_cap_steps = rast_steps;
-    a.src_ptr = d_src_ptr.as<uint32_t>();
+    if (d_row_hi.p) {
+        a.row_lo = d_src_ptr.as<uint32_t>();
+        a.row_hi = d_row_hi.as<uint32_t>();
+    } else {
+        a.row_lo = d_src_ptr.as<uint32_t>();
+        a.row_hi = d_src_ptr.as<uint32_t>() + 1;
+    }
+    a.wcnt = d_wcnt.as<uint32_t>();
+    a.n_win = n_win;
     a.syn = d_syn.as<uint32_t>();
     a.delay16 = d_delay16.as<uint16_t>();
     a.w = d_w.p;
@@ -560,7 +568,7 @@ int Arena::upload_stim(uint64_t t_arena, uint32_t k) {
 }
 
 int Arena::run(uint64_t n_steps) {
-    if (n_total == 0 || n_steps == 0) {
+    if (n_total == 0) {
         t += n_steps;
         last_run_ms = 0.0;
         return NH_OK;
@@ -576,6 +584,7 @@ int Arena::run(uint64_t n_steps) {
         if (graph_steps > stim_cap) use_graph = 0;
     }
     if (use_graph && graph_dirty) TRY(capture_graph());
+    if (n_steps == 0) return NH_OK;  // a zero-step run only prepares the graph
     HIPCHK(hipEventRecord(ev0, stream));
     while (n_steps) {
         uint64_t k = std::min<uint64_t>(n_steps, stim_cap);

@@ -122,6 +122,8 @@ class Arena {
     DBuf d_rast_ids, d_rast_off, d_rast_ctl;
     DBuf d_src_ptr, d_syn, d_delay16, d_w, d_totals;
     DBuf d_watch, d_xrec;
+    DBuf d_row_hi, d_wcnt;     // served arena: explicit row ends, window counters
+    uint32_t n_win = 0;
     float *h_stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     int stage_i = 0;

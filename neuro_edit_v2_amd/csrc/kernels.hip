@@ -96,8 +96,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
     const float ext = a.ext ? a.ext[i] : 0.0f;
     uint32_t row_lo = 0, row_hi = 0;
     if (PUSH) {
-        row_lo = a.src_ptr[i];
-        row_hi = a.src_ptr[i + 1];
+        row_lo = a.row_lo[i];
+        row_hi = a.row_hi[i];
     }
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
@@ -159,6 +159,10 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
         if (spiked) {
             a.pn_total[i] += 1;
             a.last_spike[i] = t < 0xFFFFFFFEull ? (uint32_t)t : 0xFFFFFFFEu;
+            if (a.wcnt && nr.win_steps) {
+                const uint64_t wi = tl / nr.win_steps;
+                if (wi < a.n_win) a.wcnt[(size_t)wi * a.n_total + i] += 1;
+            }
         }
     }
 
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(StepArgs a) {
     uint64_t ev = 0;
     for (uint32_t q = wave; q < c; q += nwaves) {
         const uint32_t src = uniform(a.spk[(size_t)L * a.n_total + q]);
-        const uint32_t lo = a.src_ptr[src], hi = a.src_ptr[src + 1];
+        const uint32_t lo = a.row_lo[src], hi = a.row_hi[src];
         push_row<RING, PACKED>(a, lo, hi, a.slot, lane);
         ev += hi - lo;
     }

@@ -30,6 +30,8 @@ struct alignas(64) NetRec {
     uint32_t stim_base;   // first column of this network in the stimulus window
     uint32_t active;      // 0 = slot empty (served form)
     float dt;             // step, ms
+    uint32_t win_steps;   // per-window spike counters: steps per window (0 = off)
+    uint32_t pad;
 };
 static_assert(sizeof(NetRec) == 64, "NetRec is one cache line");
 
@@ -89,8 +91,14 @@ struct StepArgs {
     uint64_t *rast_ctl;          // [0] id origin, [1] step origin, [2] overflow
     uint64_t rast_cap_ids;
     uint32_t rast_cap_steps;
-    // synapses (CSR by source, rows sorted by delay then target)
-    const uint32_t *src_ptr;     // n_total+1
+    // windowed spike counters [window][n_total] (served form) or null
+    uint32_t *wcnt;
+    uint32_t n_win;
+    // synapses (CSR by source, rows sorted by delay then target); the row
+    // of neuron i is [row_lo[i], row_hi[i]) -- for a packed CSR row_hi is
+    // row_lo + 1 (the same array), the served arena keeps both
+    const uint32_t *row_lo;
+    const uint32_t *row_hi;
     const uint32_t *syn;         // packed (tgt | delay << 27) or tgt
     const uint16_t *delay16;     // unpacked delays (null when packed)
     const void *w;               // float or int32 per synapse

@@ -340,4 +340,42 @@ int nh_net_grid(uint32_t ex, uint32_t ey, uint32_t ez, double scale, double lamb
     return NH_OK;
 }
 
+struct nh_rng_stream {
+    Rng rng;
+    uint32_t n_src;
+    float p;
+    nh_rng_stream(uint64_t seed, uint32_t n, float pp) : rng(seed), n_src(n), p(pp) {}
+};
+
+int nh_rng_stream_new(uint64_t seed, uint32_t n_src, float p, nh_rng_stream **out) {
+    if (!out) return set_error(NH_ERR_ARG, "null out");
+    *out = new (std::nothrow) nh_rng_stream(seed, n_src, p);
+    return *out ? NH_OK : set_error(NH_ERR_CAPACITY, "out of host memory");
+}
+
+void nh_rng_stream_free(nh_rng_stream *s) { delete s; }
+
+// RngStream::window -- sim/src/script.rs:2867-2894
+int nh_rng_stream_window(nh_rng_stream *s, uint32_t steps, int event, const uint32_t *pairs,
+                         uint32_t n_pairs, float strength, float *out) {
+    if (!s || (!out && steps)) return set_error(NH_ERR_ARG, "null argument");
+    const double p = (double)s->p;
+    for (uint32_t t = 0; t < steps; ++t) {
+        float *row = out + (size_t)t * s->n_src;
+        for (uint32_t k = 0; k < s->n_src; ++k) row[k] = s->rng.f64() < p ? 1.0f : 0.0f;
+        if (event && strength > 0.0f) {
+            for (uint32_t q = 0; q < n_pairs; ++q) {
+                const uint32_t i = pairs[2 * q], j = pairs[2 * q + 1];
+                if (i >= s->n_src || j >= s->n_src) continue;
+                if (s->rng.f64() < (double)strength) {
+                    const float bit = s->rng.f64() < p ? 1.0f : 0.0f;
+                    row[i] = bit;
+                    row[j] = bit;
+                }
+            }
+        }
+    }
+    return NH_OK;
+}
+
 }  // extern "C"
