@@ -111,6 +111,61 @@ def bench_batched(args):
             "rasters_identical": bool(same and bev == plain_ev)}
 
 
+def bench_served(args, ring_int=True):
+    """LSM trial requests (grid liquids of edge U{10..37}, 8 bit streams, 10
+    windows x 50 ms, one of 4 stream pairs sharing bits per window, 500 ms
+    each) arriving as a Poisson process at 50 req/s; continuous batching in
+    one resident arena.  Latency = completion - arrival; a few requests are
+    re-run alone through the plain path and their window counts compared."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    from neuro_edit_v2_amd import Sim, SimConfig
+    from neuro_edit_v2_amd import workloads as W
+    from neuro_edit_v2_amd.serve import Server, window_counts
+
+    n_req, rate, dt, steps, win = args.serve_requests, 50.0, 0.5, 1000, 100
+    with ThreadPoolExecutor(8) as ex:
+        reqs = list(ex.map(lambda k: W.lsm_request(5000 + k, dt=dt), range(n_req)))
+    cfgs = [SimConfig(dt_ms=dt, synapse_tau_ms=5.0, seed=5000 + k, int_weights=ring_int,
+                      int_scale=65536.0, raster_steps=steps) for k in range(n_req)]
+    srv = Server(max_requests=256, max_delay=15, n_windows=10, chunk_steps=96,
+                 int_weights=int(ring_int), int_scale=65536.0)
+    arr = np.cumsum(np.random.default_rng(77).exponential(1.0 / rate, n_req))
+    ids = []
+    t_start = time.perf_counter()
+    for k, (net, ser, _) in enumerate(reqs):
+        delay = t_start + arr[k] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(net, cfgs[k], ser, steps, win))
+    got = [srv.wait(r, timeout_ms=120000) for r in ids]
+    st = srv.stats()
+    lat = np.array([(g[1]["done_ns"] - g[1]["submit_ns"]) * 1e-6 for g in got])
+    span = (max(g[1]["done_ns"] for g in got) - min(g[1]["submit_ns"] for g in got)) * 1e-9
+    same = True
+    if ring_int:
+        for k in range(min(3, n_req)):
+            net, ser, _ = reqs[k]
+            s = Sim(net, cfgs[k])
+            for q in range(8):
+                s.play_input(q, ser[:, q])
+            s.run_steps(steps)
+            _, c, i = s.read_spikes()
+            same = same and np.array_equal(window_counts(c, i, int(net["n"]), win, 10), got[k][0])
+            s.close()
+    srv.close()
+    return {"requests": n_req, "arrival_rate_hz": rate,
+            "neurons_mean": round(float(np.mean([r[0]["n"] for r in reqs])), 1),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+            "events_per_s": round(st["events"] / span, 1), "events": int(st["events"]),
+            "span_s": round(span, 3), "peak_in_flight": int(st["peak_active"]),
+            "gpu_busy_frac": round(st["gpu_ms"] / 1e3 / span, 4),
+            "ring": "int Q16.16" if ring_int else "float",
+            "alone_equals_mix_checked": min(3, n_req) if ring_int else 0,
+            "alone_equals_mix": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +180,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--batch-nets", type=int, default=256)
+    ap.add_argument("--skip-served", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this backend runs on one GPU; multi-GPU figures are not produced "
@@ -216,6 +273,9 @@ def main():
                "wall_s": round(cwall, 3)}
 
     batched = None if args.skip_batched else bench_batched(args)
+    served = None
+    if not args.skip_served:
+        served = {"int": bench_served(args, True), "float": bench_served(args, False)}
 
     out = {
         "metric": METRIC,
@@ -246,6 +306,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(out))
 

@@ -230,6 +230,56 @@ int nh_batch_play_input(nh_batch *b, uint32_t net, uint32_t input, const float *
                         uint64_t len);
 int nh_batch_last_run_ms(nh_batch *b, double *ms);
 
+/* ---------------- served form ----------------
+ * A request loop (a worker thread) over one resident arena: requests are
+ * admitted while neurons, synapses and slots are free (capacity fixed at
+ * creation from free HBM when left 0), keep their state and delay ring
+ * resident, and every step all in-flight requests advance together in the
+ * same launch.  A request runs `total_steps` steps from its admission and
+ * returns per-neuron spike counts for `n_windows` windows of `win_steps`.
+ * The reference runs one thread and one simulator per job with no GPU
+ * sharing (server/src/jobs.rs:106-161,226-339); this is its batched
+ * counterpart. */
+typedef struct {
+    uint64_t cap_neurons;      /* 0 = from free HBM */
+    uint64_t cap_synapses;     /* 0 = from free HBM */
+    uint64_t cap_stim;         /* stimulus entries (0 = 64 per neuron slot) */
+    uint32_t max_requests;     /* concurrent slots (<= 65535) */
+    uint32_t max_delay;        /* ring depth - 1 (<= 31) */
+    uint32_t n_inputs;         /* stimulus inputs per request */
+    uint32_t n_windows;        /* windowed spike counters per request */
+    uint32_t chunk_steps;      /* steps per scheduling round (rounded to the ring depth) */
+    int32_t int_weights;
+    float int_scale;
+    float hbm_fraction;        /* share of free HBM to use when capacities are 0 */
+} nh_server_config;
+
+typedef struct {
+    uint64_t submit_ns, admit_ns, done_ns;  /* steady-clock stamps */
+    uint64_t t0;                            /* arena step of admission */
+    uint64_t spikes;
+    uint32_t n;                             /* neurons of the request */
+} nh_request_stats;
+
+typedef struct {
+    uint64_t steps, events, spikes;
+    uint64_t admitted, completed, peak_active;
+    uint64_t cap_neurons, cap_synapses;
+    double gpu_ms;
+} nh_server_stats;
+
+typedef struct nh_server nh_server;
+void nh_server_config_default(nh_server_config *c);
+int nh_server_create(const nh_server_config *cfg, nh_server **out);
+void nh_server_destroy(nh_server *s);
+/* Copies everything it needs; stim is [stim_steps][n_inputs] (0 past it). */
+int nh_server_submit(nh_server *s, const nh_net_desc *net, const nh_config *cfg, const float *stim,
+                     uint64_t stim_steps, uint64_t total_steps, uint32_t win_steps, uint64_t *id);
+/* Blocks up to timeout_ms; counts is [n_windows][n]. NH_ERR_NONE on timeout. */
+int nh_server_wait(nh_server *s, uint64_t id, uint32_t timeout_ms, uint32_t *counts, uint64_t cap,
+                   nh_request_stats *st);
+int nh_server_get_stats(nh_server *s, nh_server_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -386,7 +386,7 @@ StepArgs Arena::args() const {
     a.t_dev = d_t.as<uint64_t>();
     a.k = 0;
     a.slot = 0;
-    a.single_net = n_nets == 1 ? 1 : 0;
+    a.single_net = (n_nets == 1 && !dynamic) ? 1 : 0;
     a.wave_net = d_wave_net.as<uint16_t>();
     a.nets = d_nets.as<NetRec>();
     a.net0 = recs[0];
@@ -401,7 +401,9 @@ StepArgs Arena::args() const {
     a.p = d_p.as<float>();
     a.mreg = d_mreg.as<uint8_t>();
     a.P = P_;
-    a.stim_ptr = d_stim_ptr.as<uint32_t>();
+    a.stim_lo = d_stim_ptr.as<uint32_t>();
+    a.stim_hi = d_stim_hi.p ? d_stim_hi.as<uint32_t>()
+                            : (d_stim_ptr.p ? d_stim_ptr.as<uint32_t>() + 1 : nullptr);
     a.stim_input = d_stim_input.as<uint32_t>();
     a.stim_w = d_stim_w.as<float>();
     a.stim_win = d_stim_win.as<float>();
@@ -413,7 +415,7 @@ StepArgs Arena::args() const {
     a.words = words;
     a.pn_total = d_pn.as<uint32_t>();
     a.last_spike = d_last.as<uint32_t>();
-    a.net_spikes = n_nets > 1 ? d_net_spikes.as<uint64_t>() : nullptr;
+    a.net_spikes = (n_nets > 1 || dynamic) ? d_net_spikes.as<uint64_t>() : nullptr;
     a.rast_ids = rast_steps ? d_rast_ids.as<uint32_t>() : nullptr;
     a.rast_off = rast_steps ? d_rast_off.as<uint32_t>() : nullptr;
     a.rast_ctl = d_rast_ctl.as<uint64_t>();
@@ -440,7 +442,8 @@ StepArgs Arena::args() const {
 }
 
 uint32_t Arena::neuron_blocks() const {
-    return std::max<uint32_t>((n_total + kBlock - 1) / kBlock, 1);
+    const uint32_t n = grid_limit ? std::min(grid_limit, n_total) : n_total;
+    return std::max<uint32_t>((n + kBlock - 1) / kBlock, 1);
 }
 
 uint32_t Arena::scat_blocks() const {

@@ -123,7 +123,10 @@ class Arena {
     DBuf d_src_ptr, d_syn, d_delay16, d_w, d_totals;
     DBuf d_watch, d_xrec;
     DBuf d_row_hi, d_wcnt;     // served arena: explicit row ends, window counters
+    DBuf d_stim_hi;            // served arena: explicit stimulus range ends
     uint32_t n_win = 0;
+    bool dynamic = false;      // served arena: slots come and go
+    uint32_t grid_limit = 0;   // served arena: launch only the occupied prefix
     float *h_stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     int stage_i = 0;

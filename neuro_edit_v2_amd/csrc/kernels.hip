@@ -111,9 +111,9 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
         const float arr = Ring<RING>::get(rv, nr.int_scale);
         const float sv = s_old + arr;
         float I = sv + ext;
-        if (a.stim_ptr) {
+        if (a.stim_lo) {
             const float *row = a.stim_win + (size_t)a.k * a.stim_cols + nr.stim_base;
-            for (uint32_t j = a.stim_ptr[i]; j < a.stim_ptr[i + 1]; ++j)
+            for (uint32_t j = a.stim_lo[i]; j < a.stim_hi[i]; ++j)
                 I += a.stim_w[j] * row[a.stim_input[j]];
         }
         float pr[kMaxParams];
@@ -248,6 +248,26 @@ __global__ void k_watch(StepArgs a) {
     if (w >= a.n_watch) return;
     const uint64_t t = *a.t_dev + a.k;
     a.xrec[(size_t)(t % a.xrec_depth) * a.n_watch + w] = a.x[a.watch[w]];
+}
+
+// Served arena: a newly admitted range starts with empty ring columns,
+// zero current and counters, and no last spike.
+__global__ void k_clear_range(StepArgs a, uint32_t base, uint32_t len) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= len) return;
+    const uint32_t i = base + j;
+    uint32_t *ring = (uint32_t *)a.ring;
+    for (uint32_t d = 0; d < a.D; ++d) ring[(size_t)d * a.n_total + i] = 0u;
+    a.s[i] = 0.0f;
+    a.pn_total[i] = 0u;
+    a.last_spike[i] = kNoSpike;
+    if (a.wcnt)
+        for (uint32_t w = 0; w < a.n_win; ++w) a.wcnt[(size_t)w * a.n_total + i] = 0u;
+}
+
+void launch_clear_range(const StepArgs &a, uint32_t base, uint32_t len, hipStream_t st) {
+    if (!len) return;
+    hipLaunchKernelGGL(k_clear_range, dim3((len + 255) / 256), dim3(256), 0, st, a, base, len);
 }
 
 // End of a chunk: advance the device step base.

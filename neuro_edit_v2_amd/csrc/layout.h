@@ -71,7 +71,8 @@ struct StepArgs {
     const uint8_t *mreg;         // per-neuron registry model id (row mode)
     uint32_t P;
     // stimulus
-    const uint32_t *stim_ptr;    // n_total+1 or null
+    const uint32_t *stim_lo;     // per-neuron stimulus entry range or null
+    const uint32_t *stim_hi;
     const uint32_t *stim_input;
     const float *stim_w;
     const float *stim_win;       // [chunk step][columns]

@@ -426,6 +426,7 @@ int nh_set_noise(nh_sim *s, float rate_hz, float kick_mv) {
     s->cfg.noise_rate_hz = rate_hz;
     s->cfg.noise_kick_mv = kick_mv;
     Arena::fill_noise(s->ar.recs[0], s->cfg);
+    s->ar.graph_dirty = true;  // the record rides in the captured kernel arguments
     return s->ar.upload_recs();
 }
 
@@ -434,6 +435,7 @@ int nh_set_noise_sd(nh_sim *s, float sd) {
     TRY(s->ar.sync());
     s->cfg.noise_sd = sd;
     Arena::fill_noise(s->ar.recs[0], s->cfg);
+    s->ar.graph_dirty = true;
     return s->ar.upload_recs();
 }
 

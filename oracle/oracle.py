@@ -350,23 +350,27 @@ class OracleSim:
     def ring_depth(self):
         return lib().or_sim_ring_depth(self.h)
 
-    def run_steps(self, steps, record=False, cap=None):
+    def run_steps(self, steps, record=False):
         """Run; with record=True returns (counts[steps], ids) -- the raster
-        as per-step counts and ascending ids per step."""
+        as per-step counts and ascending ids per step (recorded in chunks
+        whose id buffer holds the worst case, every neuron every step)."""
         if not record:
             lib().or_sim_run(self.h, steps, None, None, 0, None)
             return None
         cnt = np.zeros(max(steps, 1), np.uint32)
-        cap = cap if cap is not None else max(self.n * steps // 8, 1 << 16)
-        while True:
+        chunk = max(1, (1 << 22) // max(self.n, 1))
+        parts, done = [], 0
+        while done < steps:
+            k = min(chunk, steps - done)
+            cap = max(self.n * k, 1)
             ids = np.zeros(cap, np.uint32)
             nid = C.c_uint64()
-            # run on a scratch copy is impossible; so size generously and
-            # refuse to lose data
-            rc = lib().or_sim_run(self.h, steps, cnt.ctypes.data, ids.ctypes.data, cap, C.byref(nid))
-            if rc != 0:
-                raise RuntimeError(f"raster capacity {cap} too small ({nid.value} ids)")
-            return cnt[:steps], ids[: nid.value]
+            rc = lib().or_sim_run(self.h, k, cnt[done:].ctypes.data, ids.ctypes.data, cap,
+                                  C.byref(nid))
+            assert rc == 0
+            parts.append(ids[: nid.value])
+            done += k
+        return cnt[:steps], (np.concatenate(parts) if parts else np.zeros(0, np.uint32))
 
     def run_ms(self, ms):
         self.run_steps(int(round(ms / self.cfg.dt_ms)))

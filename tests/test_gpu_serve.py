@@ -1,0 +1,80 @@
+"""Served form: requests admitted while others run, batched into the same
+step launches; each request's result is the one it gets alone (int ring:
+bit-exact per-window spike counts)."""
+import time
+
+import numpy as np
+import pytest
+
+from neuro_edit_v2_amd import Sim, SimConfig
+from neuro_edit_v2_amd import workloads as W
+from neuro_edit_v2_amd.serve import Server, window_counts
+
+pytestmark = pytest.mark.gpu
+
+WIN, NWIN = 40, 5
+TOTAL = WIN * NWIN
+
+
+def request(seed, edge, ring=True):
+    net, series, _ = W.lsm_request(seed, dt=0.5, windows=NWIN, win_ms=WIN // 2, edge=edge)
+    cfg = SimConfig(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=10.0, noise_sd=0.5, seed=seed,
+                    int_weights=ring, int_scale=65536.0, raster_steps=TOTAL)
+    return net, cfg, series
+
+
+def alone(net, cfg, series):
+    s = Sim(net, cfg)
+    for q in range(series.shape[1]):
+        s.play_input(q, series[:, q])
+    s.run_steps(TOTAL)
+    _, c, ids = s.read_spikes()
+    return window_counts(c, ids, int(net["n"]), WIN, NWIN)
+
+
+def test_served_results_equal_alone_in_the_mix():
+    srv = Server(cap_neurons=64 * 1024, cap_synapses=2_000_000, max_requests=8, n_windows=NWIN,
+                 chunk_steps=16, int_weights=1, int_scale=65536.0)
+    reqs = [request(s, e) for s, e in [(1, 8), (2, 11), (3, 6), (4, 13), (5, 9), (6, 7), (7, 12)]]
+    ids = []
+    for k, (net, cfg, ser) in enumerate(reqs):
+        ids.append(srv.submit(net, cfg, ser, TOTAL, WIN))
+        time.sleep(0.002 * (k % 3))       # arrivals while others are in flight
+    got = [srv.wait(r) for r in ids]
+    st = srv.stats()
+    assert st["completed"] == len(reqs)
+    assert st["peak_active"] >= 2, "requests never overlapped on the device"
+    t0s = {g[1]["t0"] for g in got}
+    assert len(t0s) > 1, "every request started at the same step"
+    for (net, cfg, ser), (counts, info) in zip(reqs, got):
+        want = alone(net, cfg, ser)
+        assert np.array_equal(counts, want)
+        assert info["spikes"] == int(want.sum())
+    assert sum(int(c.sum()) for c, _ in got) > 0
+    srv.close()
+
+
+def test_admission_waits_for_capacity():
+    # room for about two of these requests at a time
+    srv = Server(cap_neurons=2 * 1024 + 64, cap_synapses=200_000, max_requests=4, n_windows=NWIN,
+                 chunk_steps=16, int_weights=1, int_scale=65536.0)
+    reqs = [request(10 + k, 10) for k in range(5)]     # 1000 neurons each
+    ids = [srv.submit(net, cfg, ser, TOTAL, WIN) for net, cfg, ser in reqs]
+    got = [srv.wait(r) for r in ids]
+    st = srv.stats()
+    assert st["completed"] == 5 and st["peak_active"] <= 2
+    admits = sorted(g[1]["t0"] for g in got)
+    assert admits[-1] > admits[0]
+    for (net, cfg, ser), (counts, _) in zip(reqs, got):
+        assert np.array_equal(counts, alone(net, cfg, ser))
+    srv.close()
+
+
+def test_request_too_large_is_refused():
+    srv = Server(cap_neurons=1024, cap_synapses=10_000, max_requests=2, n_windows=NWIN,
+                 int_weights=1)
+    net, cfg, ser = request(3, 12)
+    from neuro_edit_v2_amd._lib import NHError
+    with pytest.raises(NHError):
+        srv.submit(net, cfg, ser, TOTAL, WIN)
+    srv.close()
