@@ -95,6 +95,8 @@ int or_sim_run(or_sim *s, uint64_t steps, uint32_t *cnt_out, uint32_t *ids_out,
 uint64_t or_sim_t(const or_sim *s);
 uint64_t or_sim_spikes_total(const or_sim *s);
 uint32_t or_sim_ring_depth(const or_sim *s);
+/* measurement counter: spike x out-synapse deliveries so far */
+uint64_t or_sim_events_total(const or_sim *s);
 void or_sim_state(const or_sim *s, uint32_t slot, double *out);  /* n */
 void or_sim_syn_current(const or_sim *s, double *out);           /* s[i] */
 void or_sim_trace(const or_sim *s, double *out);

@@ -69,6 +69,7 @@ def lib():
         "or_sim_t": (u64, [vp]),
         "or_sim_spikes_total": (u64, [vp]),
         "or_sim_ring_depth": (u32, [vp]),
+        "or_sim_events_total": (u64, [vp]),
         "or_sim_state": (None, [vp, u32, f64p]),
         "or_sim_syn_current": (None, [vp, f64p]),
         "or_sim_trace": (None, [vp, f64p]),
@@ -410,6 +411,9 @@ class OracleSim:
 
     def spikes_total(self):
         return lib().or_sim_spikes_total(self.h)
+
+    def events_total(self):
+        return lib().or_sim_events_total(self.h)
 
     def spike_row(self, back):
         out = np.zeros((self.n + 63) // 64 or 1, np.uint64)

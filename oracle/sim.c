@@ -106,6 +106,10 @@ uint64_t or_sim_spikes_total(const or_sim *s)
     return s->a ? s->a->spikes_total : s->b->spikes_total;
 }
 uint32_t or_sim_ring_depth(const or_sim *s) { return s->a ? s->a->d : s->b->d; }
+uint64_t or_sim_events_total(const or_sim *s)
+{
+    return s->a ? s->a->events_total : s->b->events_total;
+}
 
 void or_sim_state(const or_sim *s, uint32_t slot, double *out)
 {

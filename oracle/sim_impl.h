@@ -53,6 +53,7 @@ typedef struct {
     int32_t **g_map;
     uint32_t **g_ls;
     uint64_t t, spikes_total;
+    uint64_t events_total;  /* measurement only: spike x out-synapse deliveries */
     uint32_t *spiked;
 } SIMT;
 
@@ -106,6 +107,7 @@ static void FN(sim_reset)(SIMT *S)
     memset(S->pn_total, 0, sizeof(uint64_t) * n);
     S->t = 0;
     S->spikes_total = 0;
+    S->events_total = 0;
     for (uint32_t g = 0; g < S->n_groups; ++g)
         for (uint32_t k = 0; k < S->g_size[g]; ++k)
             S->g_ls[g][k] = UINT32_MAX;
@@ -366,6 +368,7 @@ static uint32_t FN(sim_step)(SIMT *S)
     /* scatter (cpu.rs:264-278) */
     for (uint32_t a = 0; a < ns; ++a) {
         uint32_t i = S->spiked[a];
+        S->events_total += S->src_ptr[i + 1] - S->src_ptr[i];
         for (uint32_t j = S->src_ptr[i]; j < S->src_ptr[i + 1]; ++j) {
             uint32_t dst = (uint32_t)((t + S->delay[j]) % S->d);
             uint32_t xid = S->xmap[j];
