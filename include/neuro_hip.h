@@ -76,6 +76,7 @@ typedef struct {
     int32_t step_shape;        /* -1 auto, 0 two kernels per step (neuron, scatter),
                                   1 one fused kernel per step (delay-1 pulled) */
     uint32_t graph_steps;      /* steps per captured graph (0 = 64) */
+    uint32_t block;            /* workgroup size of the step kernels, 64..512 (0 = 256) */
 } nh_config;
 
 /* SimConfig::default() values plus neutral additions. */
@@ -174,6 +175,12 @@ int nh_last_run_ms(nh_sim *s, double *ms);
 int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *main_n,
                      double *aux_ms, uint64_t *aux_n);
 
+/* Autotuner (addition): from step 0, time every candidate launch shape
+ * (one fused kernel or two kernels per step) x workgroup size (64..512) x
+ * captured-graph length over trial_steps, keep the fastest and reset the
+ * sim.  A text report of the trials goes to report (may be NULL). */
+int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_cap);
+
 /* ---------------- network generators (net-format restated) ----------------
  * An nh_net owns the flat arrays; nh_net_view fills an nh_net_desc that
  * points into it. */
@@ -229,6 +236,8 @@ int nh_batch_read_spikes(nh_batch *b, uint32_t net, uint64_t *step0, uint32_t *c
 int nh_batch_play_input(nh_batch *b, uint32_t net, uint32_t input, const float *series,
                         uint64_t len);
 int nh_batch_last_run_ms(nh_batch *b, double *ms);
+/* nh_autotune for a batch (from step 0; resets the batch). */
+int nh_batch_autotune(nh_batch *b, uint32_t trial_steps, char *report, uint32_t report_cap);
 
 /* ---------------- served form ----------------
  * A request loop (a worker thread) over one resident arena: requests are
@@ -262,7 +271,7 @@ typedef struct {
 } nh_request_stats;
 
 typedef struct {
-    uint64_t steps, events, spikes;
+    uint64_t steps, events, spikes;       /* events/spikes of completed requests */
     uint64_t admitted, completed, peak_active;
     uint64_t cap_neurons, cap_synapses;
     double gpu_ms;

@@ -36,6 +36,7 @@ class Config(C.Structure):
         ("spike_record_steps", C.c_uint32), ("int_weights", C.c_int32),
         ("int_scale", C.c_float), ("raster_steps", C.c_uint32),
         ("exec_mode", C.c_int32), ("step_shape", C.c_int32), ("graph_steps", C.c_uint32),
+        ("block", C.c_uint32),
     ]
 
 
@@ -99,6 +100,8 @@ def lib():
         "nh_read_spikes": (i32, [vp, P(u64), vp, u64, vp, u64, P(u64), P(u64)]),
         "nh_last_run_ms": (i32, [vp, P(f64)]),
         "nh_profile_steps": (i32, [vp, u64, P(f64), P(u64), P(f64), P(u64)]),
+        "nh_autotune": (i32, [vp, u32, C.c_char_p, u32]),
+        "nh_batch_autotune": (i32, [vp, u32, C.c_char_p, u32]),
         "nh_net_free": (None, [vp]),
         "nh_net_view": (i32, [vp, P(NetDesc)]),
         "nh_net_bench": (i32, [u32, u32, u64, f32, f32, f32, P(vp)]),

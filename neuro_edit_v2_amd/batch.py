@@ -80,6 +80,11 @@ class Batch:
                                          L.ptr(ids), ni.value, C.byref(ns), C.byref(ni)))
         return s0.value, cnt[: ns.value], ids[: ni.value]
 
+    def autotune(self, trial_steps=256):
+        buf = C.create_string_buffer(4096)
+        L.check(L.lib().nh_batch_autotune(self._h, trial_steps, buf, 4096))
+        return buf.value.decode()
+
     def last_run_ms(self):
         v = C.c_double()
         L.check(L.lib().nh_batch_last_run_ms(self._h, C.byref(v)))

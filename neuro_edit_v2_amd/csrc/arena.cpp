@@ -355,6 +355,11 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     h_src_ptr_ = std::move(src_ptr);
     HIPCHK(hipDeviceSynchronize());
 
+    if (tune.block) {
+        if (tune.block % 64 || tune.block > (uint32_t)kMaxBlock)
+            return set_error(NH_ERR_ARG, "block must be a multiple of 64 up to 512");
+        block = tune.block;
+    }
     use_graph = tune.use_graph >= 0 ? tune.use_graph : 1;
     graph_steps_req = tune.graph_steps ? tune.graph_steps : 64;
     graph_steps = 0;
@@ -382,6 +387,7 @@ StepArgs Arena::args() const {
     StepArgs a;
     std::memset(&a, 0, sizeof(a));
     a.n_total = n_total;
+    a.block = block;
     a.D = D;
     a.t_dev = d_t.as<uint64_t>();
     a.k = 0;
@@ -443,12 +449,12 @@ StepArgs Arena::args() const {
 
 uint32_t Arena::neuron_blocks() const {
     const uint32_t n = grid_limit ? std::min(grid_limit, n_total) : n_total;
-    return std::max<uint32_t>((n + kBlock - 1) / kBlock, 1);
+    return std::max<uint32_t>((n + block - 1) / block, 1);
 }
 
 uint32_t Arena::scat_blocks() const {
     if (scatter_blocks) return scatter_blocks;
-    return std::min<uint32_t>(std::max<uint32_t>(neuron_blocks(), 1), 2048);
+    return std::min<uint32_t>(std::max<uint32_t>(neuron_blocks(), 1), 2048 * 256 / block);
 }
 
 bool Arena::list_on() const { return !fused || rast_steps != 0; }
@@ -645,6 +651,50 @@ int Arena::reset() {
     return NH_OK;
 }
 
+int Arena::autotune(uint32_t trial_steps, std::string &report) {
+    if (t != 0) return set_error(NH_ERR_ARG, "autotune runs from the initial state (step 0)");
+    if (n_total == 0 || dynamic) return NH_OK;
+    struct Cand {
+        int fused;
+        uint32_t block, graph;
+        double ms;
+    };
+    std::vector<Cand> cands;
+    for (int f : {1, 0})
+        for (uint32_t b : {64u, 128u, 256u, 512u})
+            for (uint32_t g : {64u, 256u}) {
+                if (!use_graph && g != 64u) continue;
+                cands.push_back({f, b, g, 0.0});
+            }
+    Cand best{fused, block, graph_steps_req, 1e30};
+    report.clear();
+    for (Cand &c : cands) {
+        fused = c.fused;
+        block = c.block;
+        graph_steps_req = c.graph;
+        graph_dirty = true;
+        TRY(reset());
+        TRY(run(0));                       // capture outside the timed trial
+        TRY(run(std::min<uint32_t>(trial_steps, 2 * D)));  // warm caches
+        TRY(sync());
+        TRY(run(trial_steps));
+        TRY(sync());
+        c.ms = last_run_ms;
+        report += (c.fused ? "fused" : "two-kernel") + std::string(" block ") +
+                  std::to_string(c.block) + " graph " + std::to_string(graph_steps) + ": " +
+                  std::to_string(c.ms * 1e3 / trial_steps) + " us/step\n";
+        if (c.ms < best.ms) best = c;
+    }
+    fused = best.fused;
+    block = best.block;
+    graph_steps_req = best.graph;
+    graph_dirty = true;
+    TRY(reset());
+    report += "chosen: " + std::string(best.fused ? "fused" : "two-kernel") + " block " +
+              std::to_string(best.block) + " graph " + std::to_string(best.graph) + "\n";
+    return NH_OK;
+}
+
 int Arena::read_x(std::vector<float2> &out) {
     TRY(sync());
     out.resize(n_total);
@@ -673,21 +723,34 @@ int Arena::read_u64(const DBuf &b, std::vector<uint64_t> &out, size_t n) {
     return NH_OK;
 }
 
+// Totals follow from the per-neuron lifetime counts: spikes = sum pn,
+// events = sum pn[i] * out-degree(i) (every spike delivers its whole row).
 int Arena::spikes_total(uint64_t &out) {
     out = 0;
     if (t == 0 || n_total == 0) return NH_OK;
-    std::vector<uint64_t> tot;
-    TRY(read_u64(d_totals, tot, 8));
-    out = tot[0];
+    std::vector<uint32_t> pn;
+    TRY(read_u32(d_pn, pn, n_total));
+    for (uint32_t v : pn) out += v;
+    return NH_OK;
+}
+
+int Arena::spikes_in(uint32_t base, uint32_t n, uint64_t &out) {
+    out = 0;
+    if (n == 0) return NH_OK;
+    TRY(sync());
+    std::vector<uint32_t> pn(n);
+    HIPCHK(hipMemcpy(pn.data(), d_pn.as<uint32_t>() + base, (size_t)n * 4, hipMemcpyDeviceToHost));
+    for (uint32_t v : pn) out += v;
     return NH_OK;
 }
 
 int Arena::events_total(uint64_t &out) {
     out = 0;
     if (t == 0 || n_total == 0) return NH_OK;
-    std::vector<uint64_t> tot;
-    TRY(read_u64(d_totals, tot, 8));
-    out = tot[1];
+    std::vector<uint32_t> pn;
+    TRY(read_u32(d_pn, pn, n_total));
+    for (uint32_t i = 0; i < n_total; ++i)
+        if (pn[i]) out += (uint64_t)pn[i] * (h_src_ptr_[i + 1] - h_src_ptr_[i]);
     return NH_OK;
 }
 

@@ -60,6 +60,7 @@ struct Tuning {
     int use_graph = -1;      // -1 auto
     uint32_t graph_steps = 64;
     uint32_t scatter_blocks = 0;  // 0 auto
+    uint32_t block = 0;           // 0 = kBlock
 };
 
 class Arena {
@@ -89,6 +90,7 @@ class Arena {
     int read_u32(const DBuf &b, std::vector<uint32_t> &out, size_t n);
     int read_u64(const DBuf &b, std::vector<uint64_t> &out, size_t n);
     int spikes_total(uint64_t &out);
+    int spikes_in(uint32_t base, uint32_t n, uint64_t &out);
     int events_total(uint64_t &out);
     int drain_raster();   // device raster window -> host vectors
 
@@ -127,6 +129,10 @@ class Arena {
     uint32_t n_win = 0;
     bool dynamic = false;      // served arena: slots come and go
     uint32_t grid_limit = 0;   // served arena: launch only the occupied prefix
+    uint32_t block = kBlock;   // workgroup size of the step kernels
+    // Time each (shape, workgroup, graph chunk) candidate over trial_steps
+    // from the initial state, keep the fastest, then reset (t must be 0).
+    int autotune(uint32_t trial_steps, std::string &report);
     float *h_stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     int stage_i = 0;

@@ -98,6 +98,7 @@ int nh_batch_create(uint32_t n_nets, const nh_net_desc *nets, const nh_config *c
     tune.fused = cfgs[0].step_shape;
     tune.use_graph = cfgs[0].exec_mode == 0 ? -1 : (cfgs[0].exec_mode == 2 ? 1 : 0);
     tune.graph_steps = cfgs[0].graph_steps ? cfgs[0].graph_steps : 64;
+    tune.block = cfgs[0].block;
     TRY(b->ar.build(n_nets, nets, cfgs, cfgs[0].raster_steps, tune));
     b->cfgs.assign(cfgs, cfgs + n_nets);
     b->series.resize(n_nets);
@@ -133,14 +134,24 @@ int nh_batch_last_run_ms(nh_batch *b, double *ms) {
     return NH_OK;
 }
 
+int nh_batch_autotune(nh_batch *b, uint32_t trial_steps, char *report, uint32_t report_cap) {
+    CHECK_B(b);
+    std::string rep;
+    TRY(b->ar.autotune(trial_steps ? trial_steps : 256, rep));
+    for (auto &c : b->r_cnt) c.clear();
+    for (auto &v : b->r_ids) v.clear();
+    std::fill(b->r_step0.begin(), b->r_step0.end(), 0);
+    if (report && report_cap) {
+        std::strncpy(report, rep.c_str(), report_cap - 1);
+        report[report_cap - 1] = 0;
+    }
+    return NH_OK;
+}
+
 int nh_batch_spikes_total(nh_batch *b, uint32_t net, uint64_t *out) {
     CHECK_B(b);
     if (net >= b->ar.n_nets) return set_error(NH_ERR_ARG, "net out of range");
-    std::vector<uint64_t> v;
-    if (b->ar.n_nets == 1) return b->ar.spikes_total(*out);
-    TRY(b->ar.read_u64(b->ar.d_net_spikes, v, b->ar.n_nets));
-    *out = v[net];
-    return NH_OK;
+    return b->ar.spikes_in(b->ar.slots[net].base, b->ar.slots[net].n, *out);
 }
 
 int nh_batch_events_total(nh_batch *b, uint64_t *out) {

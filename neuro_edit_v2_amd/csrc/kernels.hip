@@ -77,10 +77,10 @@ __device__ __forceinline__ void push_row(const StepArgs &a, uint32_t lo, uint32_
 // rows, 2 global row table.  MMODE: 0 izh, 1 adex, 2 mixed.
 // LIST: keep the per-step spike list (two-kernel shape or raster on).
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LIST>
-__global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
+__global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     using RT = typename Ring<RING>::T;
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = blockIdx.x * (kBlock / kWave) + uniform(threadIdx.x >> 6);
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
     if (wave >= a.words) return;
     const uint32_t i = wave * kWave + lane;
     const uint32_t slot = a.slot;
@@ -183,12 +183,11 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
         }
     }
     if (!mask) return;
-    const uint32_t nsp = (uint32_t)__popcll(mask);
-    if (lane == 0) {
-        atomicAdd((unsigned long long *)&a.totals[0], (unsigned long long)nsp);
-        if (a.net_spikes) atomicAdd((unsigned long long *)&a.net_spikes[net], (unsigned long long)nsp);
-    }
+    // Spike and event totals are not counted here: they follow exactly from
+    // pn_total and the out-degrees on readback, so no wave contends for a
+    // shared counter.
     if (LIST) {
+        const uint32_t nsp = (uint32_t)__popcll(mask);
         uint32_t off = 0;
         if (lane == 0) off = atomicAdd(&a.cnt[L], nsp);
         off = uniform(off);
@@ -208,37 +207,32 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
     }
     if (PUSH) {
         // push the rows of this wave's spikes, one spike at a time, all lanes
-        uint64_t m = mask, ev = 0;
+        uint64_t m = mask;
         while (m) {
             const uint32_t b = (uint32_t)__builtin_ctzll(m);
             m &= m - 1;
             const uint32_t lo = __builtin_amdgcn_readlane(row_lo, b);
             const uint32_t hi = __builtin_amdgcn_readlane(row_hi, b);
             push_row<RING, PACKED>(a, lo, hi, slot, lane);
-            ev += hi - lo;
         }
-        if (lane == 0 && ev) atomicAdd((unsigned long long *)&a.totals[1], (unsigned long long)ev);
     }
 }
 
 // ---------------------------------------------------------------- scatter
 // Two-kernel shape: one wave per spike of step t, all synapses of the row.
 template <int RING, bool PACKED>
-__global__ __launch_bounds__(kBlock) void k_scatter(StepArgs a) {
+__global__ __launch_bounds__(kMaxBlock) void k_scatter(StepArgs a) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = blockIdx.x * (kBlock / kWave) + uniform(threadIdx.x >> 6);
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
     const uint64_t t = *a.t_dev + a.k;
     const uint32_t L = (uint32_t)(t % 3);
     const uint32_t c = a.cnt[L];
-    const uint32_t nwaves = gridDim.x * (kBlock / kWave);
-    uint64_t ev = 0;
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
     for (uint32_t q = wave; q < c; q += nwaves) {
         const uint32_t src = uniform(a.spk[(size_t)L * a.n_total + q]);
         const uint32_t lo = a.row_lo[src], hi = a.row_hi[src];
         push_row<RING, PACKED>(a, lo, hi, a.slot, lane);
-        ev += hi - lo;
     }
-    if (lane == 0 && ev) atomicAdd((unsigned long long *)&a.totals[1], (unsigned long long)ev);
 }
 
 // Watched-neuron state rows (sim/src/kernels.rs:141-147 moved out of the
@@ -276,8 +270,8 @@ __global__ void k_advance(uint64_t *t_dev, uint32_t k) { *t_dev += k; }
 // ------------------------------------------------------------ dispatch
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LIST>
 static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LIST>), dim3(blocks), dim3(kBlock),
-                       0, st, a);
+    hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LIST>), dim3(blocks),
+                       dim3(a.block), 0, st, a);
 }
 
 template <int RING, int PMODE, int MMODE>
@@ -321,12 +315,13 @@ void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mm
 }
 
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st) {
+    const dim3 b(a.block);
     if (ring == 0) {
-        if (packed) hipLaunchKernelGGL((k_scatter<0, true>), dim3(blocks), dim3(kBlock), 0, st, a);
-        else hipLaunchKernelGGL((k_scatter<0, false>), dim3(blocks), dim3(kBlock), 0, st, a);
+        if (packed) hipLaunchKernelGGL((k_scatter<0, true>), dim3(blocks), b, 0, st, a);
+        else hipLaunchKernelGGL((k_scatter<0, false>), dim3(blocks), b, 0, st, a);
     } else {
-        if (packed) hipLaunchKernelGGL((k_scatter<1, true>), dim3(blocks), dim3(kBlock), 0, st, a);
-        else hipLaunchKernelGGL((k_scatter<1, false>), dim3(blocks), dim3(kBlock), 0, st, a);
+        if (packed) hipLaunchKernelGGL((k_scatter<1, true>), dim3(blocks), b, 0, st, a);
+        else hipLaunchKernelGGL((k_scatter<1, false>), dim3(blocks), b, 0, st, a);
     }
 }
 

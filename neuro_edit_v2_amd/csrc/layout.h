@@ -11,7 +11,8 @@
 namespace nh {
 
 constexpr int kWave = 64;
-constexpr int kBlock = 256;          // 4 waves per workgroup
+constexpr int kBlock = 256;          // default workgroup: 4 waves
+constexpr int kMaxBlock = 512;       // largest workgroup the step kernels accept
 constexpr int kMaxParams = 10;       // AdEx row width
 constexpr uint32_t kNoSpike = 0xFFFFFFFFu;
 
@@ -49,6 +50,7 @@ constexpr int kKargRows = 4;        // parameter rows carried in the kernel argu
 // Kernel arguments of one step.  Pointers may be null when a feature is off.
 struct StepArgs {
     uint32_t n_total;            // arena neurons (multiple of 64)
+    uint32_t block;              // workgroup size (64..512, multiple of 64)
     uint32_t D;                  // ring depth (max over packed networks)
     const uint64_t *t_dev;       // arena step of chunk start (device)
     uint32_t k;                  // step offset within the chunk

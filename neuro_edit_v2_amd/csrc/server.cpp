@@ -313,7 +313,6 @@ int admit(nh_server *s, Request *r) {
     // ring columns, s, counters of the range start at zero
     nh::StepArgs a = ar.args();
     nh::launch_clear_range(a, (uint32_t)b, r->n_al, st);
-    HIPCHK(hipMemsetAsync(ar.d_net_spikes.as<uint64_t>() + r->slot, 0, 8, st));
     // per-request record; its local step 0 is the next arena step
     nh::NetRec &rec = ar.recs[r->slot];
     std::memset(&rec, 0, sizeof(rec));
@@ -396,14 +395,22 @@ int retire(nh_server *s, Request *r) {
         HIPCHK(hipMemcpy2DAsync(r->counts.data(), (size_t)r->n * 4, ar.d_wcnt.as<uint32_t>() + r->base,
                                 (size_t)ar.n_total * 4, (size_t)r->n * 4, nw, hipMemcpyDeviceToHost,
                                 ar.stream));
-    uint64_t sp = 0;
-    HIPCHK(hipMemcpyAsync(&sp, ar.d_net_spikes.as<uint64_t>() + r->slot, 8, hipMemcpyDeviceToHost,
-                          ar.stream));
+    // spikes and events of the request from its per-neuron lifetime counts
+    std::vector<uint32_t> pn(r->n);
+    if (r->n)
+        HIPCHK(hipMemcpyAsync(pn.data(), ar.d_pn.as<uint32_t>() + r->base, (size_t)r->n * 4,
+                              hipMemcpyDeviceToHost, ar.stream));
     nh::NetRec &rec = ar.recs[r->slot];
     rec.active = 0;
     TRY(put(ar.d_nets, r->slot, &rec, 1, ar.stream));
     HIPCHK(hipStreamSynchronize(ar.stream));
+    uint64_t sp = 0, ev = 0;
+    for (uint32_t i = 0; i < r->n; ++i) {
+        sp += pn[i];
+        ev += (uint64_t)pn[i] * (r->row_off[i + 1] - r->row_off[i]);
+    }
     r->spikes = sp;
+    s->st.events += ev;
     s->fn.release(r->base / 64, r->n_al / 64);
     s->fs.release(r->sbase, r->E);
     s->fst.release(r->stbase, r->n_stim);
@@ -640,11 +647,7 @@ int nh_server_wait(nh_server *s, uint64_t id, uint32_t timeout_ms, uint32_t *cou
 int nh_server_get_stats(nh_server *s, nh_server_stats *out) {
     if (!s || !out) return set_error(NH_ERR_ARG, "null argument");
     std::lock_guard<std::mutex> lk(s->mu);
-    *out = s->st;
-    // events: the arena's device counter (read between rounds)
-    uint64_t ev = 0;
-    if (hipMemcpy(&ev, s->ar.d_totals.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost) == hipSuccess)
-        out->events = ev;
+    *out = s->st;  // events and spikes of completed requests
     return NH_OK;
 }
 

@@ -108,6 +108,7 @@ void nh_config_default(nh_config *c) {
     c->exec_mode = 0;
     c->step_shape = -1;
     c->graph_steps = 0;
+    c->block = 0;
 }
 
 int nh_available(void) {
@@ -133,6 +134,7 @@ int nh_create(const nh_net_desc *net, const nh_config *cfg, nh_sim **out) {
     tune.fused = cfg->step_shape;
     tune.use_graph = cfg->exec_mode == 0 ? -1 : (cfg->exec_mode == 2 ? 1 : 0);
     tune.graph_steps = cfg->graph_steps ? cfg->graph_steps : 64;
+    tune.block = cfg->block;
     TRY(s->ar.build(1, net, cfg, cfg->raster_steps, tune));
     s->constants.assign(s->n_inputs, 0.0f);
     s->series.assign(s->n_inputs, {});
@@ -193,6 +195,19 @@ int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *mai
     CHECK_SIM(s);
     TRY(s->ar.profile(n_steps, *main_ms, *main_n, *aux_ms, *aux_n));
     if (!s->v_watch.empty()) s->vrec_valid += n_steps;
+    return NH_OK;
+}
+
+int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_cap) {
+    CHECK_SIM(s);
+    std::string rep;
+    TRY(s->ar.autotune(trial_steps ? trial_steps : 256, rep));
+    s->vrec_valid = 0;
+    s->grid_epoch = 0;
+    if (report && report_cap) {
+        std::strncpy(report, rep.c_str(), report_cap - 1);
+        report[report_cap - 1] = 0;
+    }
     return NH_OK;
 }
 

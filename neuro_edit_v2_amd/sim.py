@@ -39,6 +39,7 @@ class SimConfig:
     exec_mode: int = 0
     step_shape: int = -1
     graph_steps: int = 0
+    block: int = 0
 
     def to_c(self):
         c = L.Config()
@@ -245,6 +246,13 @@ class Sim:
         L.check(L.lib().nh_profile_steps(self._h, int(n), C.byref(a), C.byref(na), C.byref(b),
                                          C.byref(nb)))
         return a.value, na.value, b.value, nb.value
+
+    def autotune(self, trial_steps=256):
+        """Pick the fastest launch shape / workgroup / graph length from
+        step 0 (the sim is reset afterwards); returns the trial report."""
+        buf = C.create_string_buffer(4096)
+        L.check(L.lib().nh_autotune(self._h, trial_steps, buf, 4096))
+        return buf.value.decode()
 
     def last_run_ms(self):
         v = C.c_double()

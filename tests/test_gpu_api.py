@@ -326,3 +326,31 @@ def test_dev_mem_and_time():
     assert g.dev_mem_bytes() > 1000 * 20 * 8
     g.run_ms(7.0)
     assert g.step_count() == 14 and abs(g.time_ms() - 7.0) < 1e-9
+
+
+def test_autotune_keeps_results_bit_exact():
+    """The autotuner times launch shapes and workgroup sizes from step 0 and
+    resets; whatever it picks, the raster stays the oracle's."""
+    net = N.cube_net(5000, mean_deg=60.0, seed=21)
+    kw = dict(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=30.0, noise_sd=1.0, seed=3,
+              int_weights=True)
+    g = Sim(net, SimConfig(raster_steps=300, **kw))
+    rep = g.autotune(64)
+    assert "chosen" in rep and g.step_count() == 0
+    g.run_steps(300)
+    _, c, ids = g.read_spikes()
+    o = O.OracleSim(net, O.default_cfg(**kw))
+    rc, ri = o.run_steps(300, record=True)
+    assert np.array_equal(c, rc) and np.array_equal(ids, ri)
+
+
+@pytest.mark.parametrize("shape", [0, 1], ids=["two_kernel", "fused"])
+@pytest.mark.parametrize("block", [64, 128, 512])
+def test_workgroup_sizes_bit_exact(block, shape):
+    net = N.cube_net(4000, mean_deg=60.0, seed=22)
+    kw = dict(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=30.0, seed=5, int_weights=True)
+    g = Sim(net, SimConfig(raster_steps=200, block=block, step_shape=shape, **kw))
+    g.run_steps(200)
+    _, c, ids = g.read_spikes()
+    rc, ri = O.OracleSim(net, O.default_cfg(**kw)).run_steps(200, record=True)
+    assert np.array_equal(c, rc) and np.array_equal(ids, ri)
