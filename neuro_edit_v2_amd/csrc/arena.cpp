@@ -307,7 +307,9 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     TRY(alloc(d_s, (size_t)n_total * 4));
     TRY(alloc(d_ring, (size_t)D * n_total * 4));
     TRY(alloc(d_ext, (size_t)n_total * 4));
-    if (pmode == 0) {
+    if (pmode == 0 || pmode == 2) {
+        // row index per neuron + the rows (mode 0 also carries the rows in
+        // the kernel arguments; mode 2 reads them from this table)
         TRY(upload(d_prow, prow));
         TRY(upload(d_ptab, ptab));
     } else {
@@ -459,10 +461,65 @@ uint32_t Arena::scat_blocks() const {
 
 bool Arena::list_on() const { return !fused || rast_steps != 0; }
 
+// Every pointer a kernel variant dereferences must be set for that
+// variant; a mismatch between the build and the launch mode is reported
+// here instead of reaching the device.
+int Arena::check_args(const StepArgs &a) const {
+    auto need = [](const void *p, const char *what) -> int {
+        return p ? NH_OK : set_error(NH_ERR_ARG, std::string("step argument missing: ") + what);
+    };
+    TRY(need(a.t_dev, "t_dev"));
+    TRY(need(a.x, "x"));
+    TRY(need(a.s, "s"));
+    TRY(need(a.ring, "ring"));
+    TRY(need(a.pn_total, "pn_total"));
+    TRY(need(a.last_spike, "last_spike"));
+    TRY(need(a.row_lo, "row_lo"));
+    TRY(need(a.row_hi, "row_hi"));
+    TRY(need(a.spk, "spk"));
+    TRY(need(a.cnt, "cnt"));
+    TRY(need(a.totals, "totals"));
+    TRY(need(a.rast_ctl, "rast_ctl"));
+    if (!a.single_net) {
+        TRY(need(a.wave_net, "wave_net"));
+        TRY(need(a.nets, "nets"));
+    }
+    if (n_syn) {
+        TRY(need(a.syn, "syn"));
+        TRY(need(a.w, "w"));
+        if (!packed) TRY(need(a.delay16, "delay16"));
+    }
+    if (pmode == 0 && a.n_krows > 1) TRY(need(a.prow, "prow"));
+    if (pmode == 0 && (a.n_krows < 1 || a.n_krows > (uint32_t)kKargRows))
+        return set_error(NH_ERR_ARG, "kernel-argument parameter rows out of range");
+    if (pmode == 2) {
+        TRY(need(a.prow, "prow"));
+        TRY(need(a.ptab, "ptab"));
+    }
+    if (pmode == 1) {
+        TRY(need(a.p, "p"));
+        if (mmode == 2) TRY(need(a.mreg, "mreg"));
+    }
+    if (a.stim_lo) {
+        TRY(need(a.stim_hi, "stim_hi"));
+        TRY(need(a.stim_input, "stim_input"));
+        TRY(need(a.stim_w, "stim_w"));
+        TRY(need(a.stim_win, "stim_win"));
+    }
+    if (a.n_watch) {
+        TRY(need(a.watch, "watch"));
+        TRY(need(a.xrec, "xrec"));
+    }
+    if (a.block % 64 || a.block == 0 || a.block > (uint32_t)kMaxBlock)
+        return set_error(NH_ERR_ARG, "bad workgroup size");
+    return NH_OK;
+}
+
 // Enqueue k steps starting at arena step t0 (t0 known to the host; the
 // kernels read the absolute step from d_t, which holds t0 at this point).
 int Arena::launch_chunk(uint64_t t0, uint32_t k) {
     StepArgs a = args();
+    TRY(check_args(a));
     const uint32_t nb = neuron_blocks(), sb = scat_blocks();
     const bool list = list_on();
     for (uint32_t j = 0; j < k; ++j) {
@@ -503,6 +560,7 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
         }
         if ((rc = upload_stim(t, (uint32_t)k))) break;
         StepArgs a = args();
+        if ((rc = check_args(a))) break;
         for (uint32_t j = 0; j < k; ++j) {
             a.k = j;
             a.slot = (uint32_t)((t + j) % D);

@@ -140,6 +140,7 @@ class Arena {
 
     StepArgs args() const;
     int launch_chunk(uint64_t t0, uint32_t k);
+    int check_args(const StepArgs &a) const;
     bool list_on() const;
     // direct launches with an event pair around every step kernel
     int profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,
