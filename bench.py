@@ -178,11 +178,21 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device copy of this many bytes (byte-counter calibration)")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--batch-nets", type=int, default=256)
     ap.add_argument("--skip-served", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
+    if args.copy_probe:
+        import ctypes
+        from neuro_edit_v2_amd import _lib
+        ms = ctypes.c_double()
+        _lib.check(_lib.lib().nh_probe_copy(args.copy_probe, 20, ctypes.byref(ms)))
+        print(json.dumps({"copy_bytes": args.copy_probe, "avg_ms": ms.value,
+                          "gb_per_s": 2 * args.copy_probe / (ms.value * 1e-3) / 1e9}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this backend runs on one GPU; multi-GPU figures are not produced "
               "(networks are independent units -- see DESIGN.md for the sharding plan)",

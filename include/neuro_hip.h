@@ -181,6 +181,12 @@ int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *mai
  * sim.  A text report of the trials goes to report (may be NULL). */
 int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_cap);
 
+/* Measurement helper (addition): copy `bytes` between two device buffers
+ * with a plain vector-load/store kernel `reps` times; *ms gets the average
+ * kernel time.  Used to calibrate HBM byte counters (a copy of known size
+ * reads and writes exactly `bytes`). */
+int nh_probe_copy(uint64_t bytes, uint32_t reps, double *ms);
+
 /* ---------------- network generators (net-format restated) ----------------
  * An nh_net owns the flat arrays; nh_net_view fills an nh_net_desc that
  * points into it. */
@@ -244,8 +250,9 @@ int nh_batch_autotune(nh_batch *b, uint32_t trial_steps, char *report, uint32_t 
  * admitted while neurons, synapses and slots are free (capacity fixed at
  * creation from free HBM when left 0), keep their state and delay ring
  * resident, and every step all in-flight requests advance together in the
- * same launch.  A request runs `total_steps` steps from its admission and
- * returns per-neuron spike counts for `n_windows` windows of `win_steps`.
+ * same launch.  A request runs exactly `total_steps` (1..2^32-1) steps from
+ * its admission (the kernel stops it at its end even mid-round) and returns
+ * per-neuron spike counts for `n_windows` windows of `win_steps`.
  * The reference runs one thread and one simulator per job with no GPU
  * sharing (server/src/jobs.rs:106-161,226-339); this is its batched
  * counterpart. */

@@ -118,10 +118,6 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         for (uint64_t j = 0; j < E; ++j) {
             if (d.tgt[j] >= d.n) return set_error(NH_ERR_ARG, "synapse target out of range");
             if (d.delay[j] < 1) return set_error(NH_ERR_ARG, "delay must be >= 1 step");
-            if (d.tau && d.tau[j] > 0.0f)
-                return set_error(NH_ERR_UNSUPPORTED,
-                                 "custom-tau synapses (tau_ms > 0) are not supported by the HIP "
-                                 "backend");
             maxd = std::max<uint32_t>(maxd, d.delay[j]);
         }
         if (d.n_inputs && d.stim_ptr) {
@@ -180,6 +176,7 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
                     return d.tgt[x] < d.tgt[y];
                 });
                 for (uint32_t j : idx) {
+                    if (d.tau && d.tau[j] > 0.0f) continue;  // custom-tau: own rows below
                     const uint32_t tg = b + d.tgt[j];
                     if (packed) syn[o] = tg | ((uint32_t)d.delay[j] << 27);
                     else {
@@ -196,7 +193,58 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         }
         src_ptr[n_total] = (uint32_t)o;
         if (o > 0xFFFFFFFFull) return set_error(NH_ERR_CAPACITY, "more than 2^32 synapses");
+        n_syn = o;
     }
+    // ---- custom-tau synapses (XSyn::build, sim/src/state.rs:245-274): slot k
+    // in CSR order per network; per-source rows (k, delay, w) for the push,
+    // target-major lists of k for the fold-in at the target
+    std::vector<uint32_t> xs_lo(n_total, 0), xs_hi(n_total, 0), xs_k, xt_cnt(n_total + 1, 0), xtgt;
+    std::vector<uint16_t> xs_d;
+    std::vector<float> xs_wf, xdecay;
+    std::vector<int32_t> xs_wq;
+    for (uint32_t k = 0; k < nn; ++k) {
+        const nh_net_desc &d = nets[k];
+        if (!d.tau) continue;
+        const uint32_t b = slots[k].base;
+        const float dt = cfgs[k].dt_ms, scale = cfgs[k].int_scale;
+        const uint32_t k0 = (uint32_t)xdecay.size();
+        // slot numbering follows CSR order j
+        std::vector<uint32_t> slot_of(d.n_syn, 0xFFFFFFFFu);
+        for (uint64_t j = 0; j < d.n_syn; ++j)
+            if (d.tau[j] > 0.0f) {
+                slot_of[j] = (uint32_t)xdecay.size();
+                const float tau = d.tau[j] > 1e-6f ? d.tau[j] : 1e-6f;
+                xdecay.push_back(std::exp(-dt / tau));
+                xtgt.push_back(b + d.tgt[j]);
+                xt_cnt[b + d.tgt[j] + 1]++;
+            }
+        (void)k0;
+        for (uint32_t i = 0; i < d.n; ++i) {
+            xs_lo[b + i] = (uint32_t)xs_k.size();
+            for (uint32_t j = d.src_ptr[i]; j < d.src_ptr[i + 1]; ++j) {
+                if (slot_of[j] == 0xFFFFFFFFu) continue;
+                xs_k.push_back(slot_of[j]);
+                xs_d.push_back(d.delay[j]);
+                if (ring_int) xs_wq.push_back(quantize(d.w[j], scale));
+                else xs_wf.push_back(d.w[j]);
+            }
+            xs_hi[b + i] = (uint32_t)xs_k.size();
+        }
+        for (uint32_t i = d.n; i < align64(d.n); ++i) xs_lo[b + i] = xs_hi[b + i] = (uint32_t)xs_k.size();
+    }
+    n_x = (uint32_t)xdecay.size();
+    std::vector<uint32_t> xt_ptr, xt_idx;
+    if (n_x) {
+        xt_ptr.assign(n_total + 1, 0);
+        for (uint32_t i = 0; i < n_total; ++i) xt_ptr[i + 1] = xt_ptr[i] + xt_cnt[i + 1];
+        xt_idx.resize(n_x);
+        std::vector<uint32_t> fill(n_total, 0);
+        for (uint32_t kx = 0; kx < n_x; ++kx) xt_idx[xt_ptr[xtgt[kx]] + fill[xtgt[kx]]++] = kx;
+    }
+    // total out-degree per neuron (events on readback)
+    h_deg_.assign(n_total, 0);
+    for (uint32_t i = 0; i < n_total; ++i)
+        h_deg_[i] = (src_ptr[i + 1] - src_ptr[i]) + (n_x ? xs_hi[i] - xs_lo[i] : 0);
     // one kernel per step (own-spike push) unless the two-kernel shape is asked for
     fused = tune.fused == 0 ? 0 : 1;
 
@@ -346,6 +394,19 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     if (ring_int) TRY(upload(d_w, wq));
     else TRY(upload(d_w, wf));
     TRY(alloc(d_totals, 8 * 8));
+    if (n_x) {
+        TRY(upload(d_xt_ptr, xt_ptr));
+        TRY(upload(d_xt_idx, xt_idx));
+        TRY(upload(d_xdecay, xdecay));
+        TRY(alloc(d_xsyn, (size_t)n_x * 4));
+        TRY(alloc(d_xring, (size_t)D * n_x * 4));
+        TRY(upload(d_xs_lo, xs_lo));
+        TRY(upload(d_xs_hi, xs_hi));
+        TRY(upload(d_xs_k, xs_k));
+        TRY(upload(d_xs_d, xs_d));
+        if (ring_int) TRY(upload(d_xs_w, xs_wq));
+        else TRY(upload(d_xs_w, xs_wf));
+    }
     rast_steps = raster_steps;
     if (rast_steps) {
         rast_cap_ids = std::min<uint64_t>((uint64_t)std::max<uint32_t>(n_total, 1) * rast_steps,
@@ -438,6 +499,17 @@ StepArgs Arena::args() const {
     }
     a.wcnt = d_wcnt.as<uint32_t>();
     a.n_win = n_win;
+    a.n_x = n_x;
+    a.xt_ptr = n_x ? d_xt_ptr.as<uint32_t>() : nullptr;
+    a.xt_idx = d_xt_idx.as<uint32_t>();
+    a.xdecay = d_xdecay.as<float>();
+    a.xsyn = d_xsyn.as<float>();
+    a.xring = d_xring.p;
+    a.xs_lo = n_x ? d_xs_lo.as<uint32_t>() : nullptr;
+    a.xs_hi = d_xs_hi.as<uint32_t>();
+    a.xs_k = d_xs_k.as<uint32_t>();
+    a.xs_d = d_xs_d.as<uint16_t>();
+    a.xs_w = d_xs_w.p;
     a.syn = d_syn.as<uint32_t>();
     a.delay16 = d_delay16.as<uint16_t>();
     a.w = d_w.p;
@@ -505,6 +577,18 @@ int Arena::check_args(const StepArgs &a) const {
         TRY(need(a.stim_input, "stim_input"));
         TRY(need(a.stim_w, "stim_w"));
         TRY(need(a.stim_win, "stim_win"));
+    }
+    if (a.n_x) {
+        TRY(need(a.xt_ptr, "xt_ptr"));
+        TRY(need(a.xt_idx, "xt_idx"));
+        TRY(need(a.xdecay, "xdecay"));
+        TRY(need(a.xsyn, "xsyn"));
+        TRY(need(a.xring, "xring"));
+        TRY(need(a.xs_lo, "xs_lo"));
+        TRY(need(a.xs_hi, "xs_hi"));
+        TRY(need(a.xs_k, "xs_k"));
+        TRY(need(a.xs_d, "xs_d"));
+        TRY(need(a.xs_w, "xs_w"));
     }
     if (a.n_watch) {
         TRY(need(a.watch, "watch"));
@@ -692,6 +776,10 @@ int Arena::reset() {
     HIPCHK(hipMemcpyAsync(d_x.p, d_x0.p, d_x0.bytes, hipMemcpyDeviceToDevice, stream));
     HIPCHK(hipMemsetAsync(d_s.p, 0, d_s.bytes, stream));
     HIPCHK(hipMemsetAsync(d_ring.p, 0, d_ring.bytes, stream));
+    if (n_x) {
+        HIPCHK(hipMemsetAsync(d_xsyn.p, 0, d_xsyn.bytes, stream));
+        HIPCHK(hipMemsetAsync(d_xring.p, 0, d_xring.bytes, stream));
+    }
     HIPCHK(hipMemsetAsync(d_cnt.p, 0, d_cnt.bytes, stream));
     HIPCHK(hipMemsetAsync(d_totals.p, 0, d_totals.bytes, stream));
     if (d_bits.p) HIPCHK(hipMemsetAsync(d_bits.p, 0, d_bits.bytes, stream));
@@ -808,7 +896,7 @@ int Arena::events_total(uint64_t &out) {
     std::vector<uint32_t> pn;
     TRY(read_u32(d_pn, pn, n_total));
     for (uint32_t i = 0; i < n_total; ++i)
-        if (pn[i]) out += (uint64_t)pn[i] * (h_src_ptr_[i + 1] - h_src_ptr_[i]);
+        if (pn[i]) out += (uint64_t)pn[i] * h_deg_[i];
     return NH_OK;
 }
 

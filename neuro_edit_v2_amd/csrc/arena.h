@@ -18,6 +18,7 @@ void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mm
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st);
 void launch_watch(const StepArgs &a, hipStream_t st);
 void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
+void launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 
 int set_error(int code, const std::string &msg);
 
@@ -126,6 +127,10 @@ class Arena {
     DBuf d_watch, d_xrec;
     DBuf d_row_hi, d_wcnt;     // served arena: explicit row ends, window counters
     DBuf d_stim_hi;            // served arena: explicit stimulus range ends
+    // custom-tau synapses
+    DBuf d_xt_ptr, d_xt_idx, d_xdecay, d_xsyn, d_xring, d_xs_lo, d_xs_hi, d_xs_k, d_xs_d, d_xs_w;
+    uint32_t n_x = 0;
+    std::vector<uint32_t> h_deg_;      // total out-degree per arena neuron
     uint32_t n_win = 0;
     bool dynamic = false;      // served arena: slots come and go
     uint32_t grid_limit = 0;   // served arena: launch only the occupied prefix

@@ -72,6 +72,22 @@ __device__ __forceinline__ void push_row(const StepArgs &a, uint32_t lo, uint32_
     }
 }
 
+// Custom-tau synapses of one spike: arrival marks in the per-synapse ring
+// (cpu.rs:271-273 pushes the synapse id; here the weight lands in its slot).
+template <int RING>
+__device__ __forceinline__ void push_custom(const StepArgs &a, uint32_t src, uint32_t slot0,
+                                            uint32_t lane) {
+    using RT = typename Ring<RING>::T;
+    RT *xr = (RT *)a.xring;
+    const RT *w = (const RT *)a.xs_w;
+    const uint32_t lo = a.xs_lo[src], hi = a.xs_hi[src];
+    for (uint32_t e = lo + lane; e < hi; e += kWave) {
+        uint32_t sl = slot0 + a.xs_d[e];
+        sl = sl >= a.D ? sl - a.D : sl;
+        Ring<RING>::add(xr + (size_t)sl * a.n_x + a.xs_k[e], w[e]);
+    }
+}
+
 // ---------------------------------------------------------------- step
 // PMODE: 0 parameter rows in the kernel arguments (<= 4 rows), 1 per-neuron
 // rows, 2 global row table.  MMODE: 0 izh, 1 adex, 2 mixed.
@@ -104,13 +120,28 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     const uint64_t t = *a.t_dev + a.k;
 
     const uint32_t local = i - nr.base;
-    const bool act = local < nr.n && nr.active;
+    const uint64_t tl = t - (uint64_t)nr.t0;
+    const bool act = local < nr.n && nr.active && (nr.t_len == 0 || tl < nr.t_len);
     bool spiked = false;
     if (act) {
         ring[ri] = (RT)0;
         const float arr = Ring<RING>::get(rv, nr.int_scale);
         const float sv = s_old + arr;
         float I = sv + ext;
+        if (a.xt_ptr) {
+            // custom-tau traces into this neuron (cpu.rs:208-228): this
+            // step's arrival, use, then decay -- each slot has one target,
+            // so its thread owns it
+            RT *xr = (RT *)a.xring + (size_t)slot * a.n_x;
+            for (uint32_t j = a.xt_ptr[i]; j < a.xt_ptr[i + 1]; ++j) {
+                const uint32_t kx = a.xt_idx[j];
+                const RT av = xr[kx];
+                xr[kx] = (RT)0;
+                const float xv = a.xsyn[kx] + Ring<RING>::get(av, nr.int_scale);
+                I += xv;
+                a.xsyn[kx] = xv * a.xdecay[kx];
+            }
+        }
         if (a.stim_lo) {
             const float *row = a.stim_win + (size_t)a.k * a.stim_cols + nr.stim_base;
             for (uint32_t j = a.stim_lo[i]; j < a.stim_hi[i]; ++j)
@@ -147,7 +178,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
         const bool is_adex = MMODE == 1 || (MMODE == 2 && model == 1);
         // sim/src/cpu.rs:240-245: kick * model gain (C for AdEx, adex.rs:189-191)
         const float gain = is_adex ? pr[0] : 1.0f;
-        const uint64_t tl = t - (uint64_t)nr.t0;
         if (nr.pkick > 0.0f && kick_fires(local, tl, nr.seed, nr.pkick)) I += nr.kick * gain;
         if (nr.sd != 0.0f) I += (nr.sd * gauss_draw(local, tl, nr.seed)) * gain;
         if (is_adex)
@@ -214,6 +244,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
             const uint32_t lo = __builtin_amdgcn_readlane(row_lo, b);
             const uint32_t hi = __builtin_amdgcn_readlane(row_hi, b);
             push_row<RING, PACKED>(a, lo, hi, slot, lane);
+            if (a.xs_lo) push_custom<RING>(a, wave * kWave + b, slot, lane);
         }
     }
 }
@@ -232,6 +263,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_scatter(StepArgs a) {
         const uint32_t src = uniform(a.spk[(size_t)L * a.n_total + q]);
         const uint32_t lo = a.row_lo[src], hi = a.row_hi[src];
         push_row<RING, PACKED>(a, lo, hi, a.slot, lane);
+        if (a.xs_lo) push_custom<RING>(a, src, a.slot, lane);
     }
 }
 
@@ -262,6 +294,18 @@ __global__ void k_clear_range(StepArgs a, uint32_t base, uint32_t len) {
 void launch_clear_range(const StepArgs &a, uint32_t base, uint32_t len, hipStream_t st) {
     if (!len) return;
     hipLaunchKernelGGL(k_clear_range, dim3((len + 255) / 256), dim3(256), 0, st, a, base, len);
+}
+
+// Plain copy of n16 16-byte words (byte-counter calibration).
+__global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
+void launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st) {
+    const uint64_t n16 = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16);
 }
 
 // End of a chunk: advance the device step base.

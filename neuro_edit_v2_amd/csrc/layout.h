@@ -32,7 +32,7 @@ struct alignas(64) NetRec {
     uint32_t active;      // 0 = slot empty (served form)
     float dt;             // step, ms
     uint32_t win_steps;   // per-window spike counters: steps per window (0 = off)
-    uint32_t pad;
+    uint32_t t_len;       // local steps the network runs (0 = unbounded)
 };
 static_assert(sizeof(NetRec) == 64, "NetRec is one cache line");
 
@@ -106,6 +106,21 @@ struct StepArgs {
     const uint16_t *delay16;     // unpacked delays (null when packed)
     const void *w;               // float or int32 per synapse
     uint64_t *totals;            // [0] spikes, [1] events, [2+L] list fill before step of list L
+    // custom-tau synapses (own exponential trace, sim/src/state.rs:230-274):
+    // per slot k a trace xsyn[k] with decay xdecay[k] and an arrival ring
+    // xring[slot][k]; target-major lists xt for the fold-in, per-source
+    // rows xs (k, delay, w) for the push
+    uint32_t n_x;
+    const uint32_t *xt_ptr;      // n_total+1 or null (no custom synapse)
+    const uint32_t *xt_idx;
+    const float *xdecay;
+    float *xsyn;
+    void *xring;                 // D x n_x (float or int32)
+    const uint32_t *xs_lo;
+    const uint32_t *xs_hi;
+    const uint32_t *xs_k;
+    const uint16_t *xs_d;
+    const void *xs_w;
     // watched neurons (record ring of state rows)
     const uint32_t *watch;
     uint32_t n_watch;

@@ -326,6 +326,8 @@ int admit(nh_server *s, Request *r) {
     rec.active = 1;
     rec.dt = r->cfg.dt_ms;
     rec.win_steps = r->win_steps;
+    // the request stops exactly after its own steps, whatever the round length
+    rec.t_len = (uint32_t)r->total_steps;
     TRY(put(ar.d_nets, r->slot, &rec, 1, st));
     HIPCHK(hipStreamSynchronize(st));
     r->t0 = ar.t;
@@ -535,6 +537,8 @@ int nh_server_submit(nh_server *s, const nh_net_desc *d, const nh_config *cfg, c
     if (d->n_inputs > s->cfg.n_inputs) return set_error(NH_ERR_ARG, "too many inputs for the server");
     if (d->n && d->n_state != 2) return set_error(NH_ERR_ARG, "state row width must be 2");
     if (d->n_params > (uint32_t)nh::kMaxParams) return set_error(NH_ERR_ARG, "param row too wide");
+    if (total_steps == 0 || total_steps > 0xFFFFFFFFull)
+        return set_error(NH_ERR_ARG, "total_steps must be 1..2^32-1");
     auto r = std::make_unique<Request>();
     r->submit_ns = now_ns();  // latency counts the request's preparation too
     r->n = d->n;

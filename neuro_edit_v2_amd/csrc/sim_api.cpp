@@ -211,6 +211,32 @@ int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_c
     return NH_OK;
 }
 
+int nh_probe_copy(uint64_t bytes, uint32_t reps, double *ms) {
+    if (!ms || bytes < 16 || reps == 0) return set_error(NH_ERR_ARG, "bad probe arguments");
+    if (!nh_available()) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
+    bytes &= ~uint64_t(15);
+    nh::DBuf a, b;
+    HIPCHK(a.alloc(bytes));
+    HIPCHK(b.alloc(bytes));
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    nh::launch_copy(a.p, b.p, bytes, st);  // warm
+    HIPCHK(hipEventRecord(e0, st));
+    for (uint32_t r = 0; r < reps; ++r) nh::launch_copy(a.p, b.p, bytes, st);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    return NH_OK;
+}
+
 static int read_last(nh_sim *s, std::vector<uint32_t> &last) {
     return s->ar.read_u32(s->ar.d_last, last, s->n);
 }

@@ -354,3 +354,53 @@ def test_workgroup_sizes_bit_exact(block, shape):
     _, c, ids = g.read_spikes()
     rc, ri = O.OracleSim(net, O.default_cfg(**kw)).run_steps(200, record=True)
     assert np.array_equal(c, rc) and np.array_equal(ids, ri)
+
+
+def lsm_like(n_side, seed):
+    """sim/tests/gpu_vs_cpu.rs:11-18"""
+    b = O.NetBuilder()
+    rng = O.Rng(seed)
+    g = b.grid3d(n_side, n_side, n_side, 60.0)
+    b.set_exc_inh_exact(g, n_side ** 3 // 5, rng)
+    b.create_connections(g, 0.4, 100.0, 5.0, rng)
+    return b.build(0.1)
+
+
+@SHAPES
+@pytest.mark.parametrize("ring", [False, True], ids=["float", "int"])
+def test_custom_tau_single_synapse(shape, ring):
+    """cpu_ref.rs:117-206: a tau=20 synapse feeds its own trace; the GPU
+    state and spikes follow the oracle bit for bit."""
+    b = line(2)
+    b.push_synapse(0, 1, 30.0, 0.2, 20.0)
+    g, o = pair_sims(b.build(0.1), shape, int_weights=ring)
+    for s in (g, o):
+        s.set_external(0, 25.0)
+        s.run_steps(40)
+        s.set_external(0, 0.0)
+        s.run_steps(200)
+    assert f32eq(g.v(), o.v()) and f32eq(g.state(1), o.state(1))
+    assert g.spikes_total() == o.spikes_total()
+
+
+def test_custom_tau_net_matches_oracle():
+    """gpu_vs_cpu.rs:51-80: per-synapse tau on the recurrent edges
+    (Gaussian 25 +- 10 ms, clamped at 0 = merged path); int ring raster and
+    state bit-exact."""
+    net = lsm_like(8, 42)
+    r = np.random.default_rng(9)
+    net["tau"] = np.maximum(r.normal(25.0, 10.0, len(net["tgt"])), 0.0).astype(np.float32)
+    net["tau"][::7] = 0.0
+    kw = dict(record_spikes=True, int_weights=True, noise_rate_hz=300.0, seed=4)
+    g = Sim(net, SimConfig(raster_steps=500, **kw))
+    o = O.OracleSim(net, O.default_cfg(**kw))
+    for i in range(8):
+        g.set_external(i, 12.0)
+        o.set_external(i, 12.0)
+    g.run_steps(500)
+    _, c, ids = g.read_spikes()
+    rc, ri = o.run_steps(500, record=True)
+    assert rc.sum() > 0
+    assert np.array_equal(c, rc) and np.array_equal(ids, ri)
+    assert f32eq(g.v(), o.v())
+    assert g.events_total() == o.events_total()
