@@ -166,6 +166,51 @@ def bench_served(args, ring_int=True):
             "alone_equals_mix": bool(same)}
 
 
+def bench_large(args, kw):
+    """The 1M-neuron network (random cube, Poisson(100) out-degree, ~100M
+    synapses), same generator, noise and ring as the headline workload,
+    400 steps after 40 warm-up steps: the step kernel at a size where the
+    chip is full, with its roofline fraction."""
+    from neuro_edit_v2_amd import Sim, SimConfig
+    from neuro_edit_v2_amd import net as N
+
+    n, steps, warm, dt = args.large_n, 400, 40, kw["dt_ms"]
+    t_gen = time.time()
+    net = N.cube_net(n, mean_deg=100.0, seed=1, w_scale=0.7)
+    t_gen = time.time() - t_gen
+    n_syn = int(len(net["tgt"]))
+    sim = Sim(net, SimConfig(exec_mode=args.exec_mode, step_shape=args.shape, **kw))
+    del net
+    sim.run_steps(warm)
+    sim.synchronize()
+    ev0, sp0 = sim.events_total(), sim.spikes_total()
+    t0 = time.perf_counter()
+    sim.run_steps(steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = sim.last_run_ms()
+    events = sim.events_total() - ev0
+    spikes = sim.spikes_total() - sp0
+    ev_step = events / steps
+    kbytes = n * B_NEURON + ev_step * B_EVENT
+    kavg_ms = gpu_ms / steps
+    achieved = kbytes / (kavg_ms * 1e-3) / 1e9
+    return {"workload": f"random-cube Izhikevich {n} neurons x Poisson(100) out-synapses, "
+                        f"dt 0.5 ms, {steps} steps, same noise and ring",
+            "neurons": n, "synapses": n_syn, "gen_s": round(t_gen, 2),
+            "dev_mem_bytes": int(sim.dev_mem_bytes()),
+            "steps": steps, "warmup": warm,
+            "events_per_s": round(events / wall, 1),
+            "sim_ms_per_s": round(steps * dt / wall, 1),
+            "ms_per_step": round(wall / steps * 1e3, 5),
+            "spikes": int(spikes), "events": int(events),
+            "rate_hz": round(spikes / n / (steps * dt / 1000.0), 3),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "avg_launch_us": round(kavg_ms * 1e3, 3),
+                         "algorithmic_bytes_per_launch": int(kbytes)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,18 +225,30 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (byte-counter calibration)")
+    ap.add_argument("--lane-bytes", type=int, default=16, choices=[4, 16])
+    ap.add_argument("--launch-probe", type=int, default=0,
+                    help="only time graph replays of empty kernels of this many blocks")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--batch-nets", type=int, default=256)
     ap.add_argument("--skip-served", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--skip-large", action="store_true")
+    ap.add_argument("--large-n", type=int, default=1000000)
     args = ap.parse_args()
-    if args.copy_probe:
+    if args.copy_probe or args.launch_probe:
         import ctypes
         from neuro_edit_v2_amd import _lib
-        ms = ctypes.c_double()
-        _lib.check(_lib.lib().nh_probe_copy(args.copy_probe, 20, ctypes.byref(ms)))
-        print(json.dumps({"copy_bytes": args.copy_probe, "avg_ms": ms.value,
-                          "gb_per_s": 2 * args.copy_probe / (ms.value * 1e-3) / 1e9}))
+        v = ctypes.c_double()
+        if args.copy_probe:
+            _lib.check(_lib.lib().nh_probe_copy(args.copy_probe, 20, args.lane_bytes, ctypes.byref(v)))
+            print(json.dumps({"copy_bytes": args.copy_probe, "lane_bytes": args.lane_bytes,
+                              "avg_ms": v.value,
+                              "gb_per_s": 2 * args.copy_probe / (v.value * 1e-3) / 1e9}))
+        if args.launch_probe:
+            for blocks in (1, args.launch_probe):
+                _lib.check(_lib.lib().nh_probe_launch(blocks, 2000, ctypes.byref(v)))
+                print(json.dumps({"empty_kernel_graph_us": v.value, "blocks": blocks,
+                                  "threads_per_block": 256}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this backend runs on one GPU; multi-GPU figures are not produced "
@@ -283,6 +340,8 @@ def main():
                "sim_ms_per_s": round(args.cpu_steps * dt / cwall, 2),
                "wall_s": round(cwall, 3)}
 
+    del sim
+    large = None if args.skip_large else bench_large(args, kw)
     batched = None if args.skip_batched else bench_batched(args)
     served = None
     if not args.skip_served:
@@ -316,6 +375,7 @@ def main():
         "gpu_event_ms": round(gpu_ms, 3),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "large": large,
         "batched": batched,
         "served": served,
     }

@@ -70,8 +70,10 @@ typedef struct {
     uint32_t spike_record_steps;
     int32_t int_weights;       /* fixed-point delay ring (sim/src/kernels.rs:57-66) */
     float int_scale;           /* 4096 = reference, 65536 = Q16.16 */
-    uint32_t raster_steps;     /* > 0: keep the full spike raster, drained every
-                                  raster_steps steps into host memory (nh_read_spikes) */
+    uint32_t raster_steps;     /* > 0: keep the full spike raster; the device holds a
+                                  window of raster_steps steps (one bit per neuron per
+                                  step), drained into host memory when it fills and on
+                                  nh_read_spikes */
     int32_t exec_mode;         /* 0 auto, 1 direct launches, 2 hipGraph chunks */
     int32_t step_shape;        /* -1 auto, 0 two kernels per step (neuron, scatter),
                                   1 one fused kernel per step (delay-1 pulled) */
@@ -181,11 +183,16 @@ int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *mai
  * sim.  A text report of the trials goes to report (may be NULL). */
 int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_cap);
 
-/* Measurement helper (addition): copy `bytes` between two device buffers
- * with a plain vector-load/store kernel `reps` times; *ms gets the average
- * kernel time.  Used to calibrate HBM byte counters (a copy of known size
- * reads and writes exactly `bytes`). */
-int nh_probe_copy(uint64_t bytes, uint32_t reps, double *ms);
+/* Measurement helpers (additions, no counterpart in the reference).
+ * nh_probe_copy: copy `bytes` between two device buffers with a plain
+ * load/store kernel of `lane_bytes` (4 or 16) per lane, `reps` times; *ms
+ * gets the average kernel time.  A copy of known size reads and writes
+ * exactly `bytes`, which calibrates the HBM byte counters.
+ * nh_probe_launch: replay a graph of `launches` empty kernels of `blocks`
+ * x 256 threads; *us gets the time per kernel (the one-launch-per-step
+ * floor). */
+int nh_probe_copy(uint64_t bytes, uint32_t reps, uint32_t lane_bytes, double *ms);
+int nh_probe_launch(uint32_t blocks, uint32_t launches, double *us);
 
 /* ---------------- network generators (net-format restated) ----------------
  * An nh_net owns the flat arrays; nh_net_view fills an nh_net_desc that

@@ -101,7 +101,8 @@ def lib():
         "nh_last_run_ms": (i32, [vp, P(f64)]),
         "nh_profile_steps": (i32, [vp, u64, P(f64), P(u64), P(f64), P(u64)]),
         "nh_autotune": (i32, [vp, u32, C.c_char_p, u32]),
-        "nh_probe_copy": (i32, [u64, u32, P(f64)]),
+        "nh_probe_copy": (i32, [u64, u32, u32, P(f64)]),
+        "nh_probe_launch": (i32, [u32, u32, P(f64)]),
         "nh_batch_autotune": (i32, [vp, u32, C.c_char_p, u32]),
         "nh_net_free": (None, [vp]),
         "nh_net_view": (i32, [vp, P(NetDesc)]),

@@ -141,6 +141,8 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     if (n_total == 0) n_total = 0;
     words = n_total / 64;
     D = std::max<uint32_t>(maxd + 1, 2);  // ring_depth, state.rs:221-224
+    // synapse indices are 32-bit on the device (and one value is reserved)
+    if (total_syn >= 0xFFFF0000ull) return set_error(NH_ERR_CAPACITY, "arena exceeds 2^32 - 2^16 synapses");
     n_syn = total_syn;
     mmode = any_adex ? (any_izh ? 2 : 1) : 0;
     stim_cols = cols;
@@ -407,13 +409,10 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         if (ring_int) TRY(upload(d_xs_w, xs_wq));
         else TRY(upload(d_xs_w, xs_wf));
     }
+    // raster window: one spike word per wave per step (no atomics; the
+    // host decodes ids, ascending, when it drains the window)
     rast_steps = raster_steps;
-    if (rast_steps) {
-        rast_cap_ids = std::min<uint64_t>((uint64_t)std::max<uint32_t>(n_total, 1) * rast_steps,
-                                          1ull << 28);
-        TRY(alloc(d_rast_ids, rast_cap_ids * 4));
-        TRY(alloc(d_rast_off, (size_t)rast_steps * 4));
-    }
+    if (rast_steps) TRY(alloc(d_rast_bits, (size_t)rast_steps * std::max<uint32_t>(words, 1) * 8));
     TRY(alloc(d_rast_ctl, 4 * 8));
     h_src_ptr_ = std::move(src_ptr);
     HIPCHK(hipDeviceSynchronize());
@@ -485,10 +484,8 @@ StepArgs Arena::args() const {
     a.pn_total = d_pn.as<uint32_t>();
     a.last_spike = d_last.as<uint32_t>();
     a.net_spikes = (n_nets > 1 || dynamic) ? d_net_spikes.as<uint64_t>() : nullptr;
-    a.rast_ids = rast_steps ? d_rast_ids.as<uint32_t>() : nullptr;
-    a.rast_off = rast_steps ? d_rast_off.as<uint32_t>() : nullptr;
+    a.rast_bits = rast_steps ? d_rast_bits.as<uint64_t>() : nullptr;
     a.rast_ctl = d_rast_ctl.as<uint64_t>();
-    a.rast_cap_ids = rast_cap_ids;
     a.rast_cap_steps = rast_steps;
     if (d_row_hi.p) {
         a.row_lo = d_src_ptr.as<uint32_t>();
@@ -530,8 +527,6 @@ uint32_t Arena::scat_blocks() const {
     if (scatter_blocks) return scatter_blocks;
     return std::min<uint32_t>(std::max<uint32_t>(neuron_blocks(), 1), 2048 * 256 / block);
 }
-
-bool Arena::list_on() const { return !fused || rast_steps != 0; }
 
 // Every pointer a kernel variant dereferences must be set for that
 // variant; a mismatch between the build and the launch mode is reported
@@ -605,11 +600,10 @@ int Arena::launch_chunk(uint64_t t0, uint32_t k) {
     StepArgs a = args();
     TRY(check_args(a));
     const uint32_t nb = neuron_blocks(), sb = scat_blocks();
-    const bool list = list_on();
     for (uint32_t j = 0; j < k; ++j) {
         a.k = j;
         a.slot = (uint32_t)((t0 + j) % D);
-        launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, list, stream);
+        launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
         if (!fused) launch_scatter(a, sb, ring_int, packed != 0, stream);
         if (n_watch) launch_watch(a, stream);
     }
@@ -631,7 +625,6 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     int rc = NH_OK;
     const uint32_t nb = neuron_blocks(), sb = scat_blocks();
-    const bool list = list_on();
     while (n_steps && rc == NH_OK) {
         uint64_t k = std::min<uint64_t>(n_steps, std::min<uint32_t>(kPool, stim_cap));
         if (rast_steps) {
@@ -649,7 +642,7 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
             a.k = j;
             a.slot = (uint32_t)((t + j) % D);
             (void)hipEventRecord(ev[4 * j], stream);
-            launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, list, stream);
+            launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
             (void)hipEventRecord(ev[4 * j + 1], stream);
             if (!fused) {
                 (void)hipEventRecord(ev[4 * j + 2], stream);
@@ -790,7 +783,7 @@ int Arena::reset() {
     HIPCHK(hipMemsetAsync(d_t.p, 0, 8, stream));
     HIPCHK(hipStreamSynchronize(stream));
     t = 0;
-    rast_id0 = rast_step0 = 0;
+    rast_step0 = 0;
     h_rast_cnt.clear();
     h_rast_ids.clear();
     h_rast_step0 = 0;
@@ -902,30 +895,31 @@ int Arena::events_total(uint64_t &out) {
 
 int Arena::drain_raster() {
     if (!rast_steps) return NH_OK;
-    uint64_t total = 0;
-    TRY(spikes_total(total));
     std::vector<uint64_t> ctl;
     TRY(read_u64(d_rast_ctl, ctl, 4));
-    if (ctl[2]) return set_error(NH_ERR_CAPACITY, "raster overflow: lower raster_steps");
+    if (ctl[2]) return set_error(NH_ERR_CAPACITY, "raster window overflow");
     const uint64_t steps = t - rast_step0;
-    const uint64_t nid = total - rast_id0;
+    if (steps > rast_steps) return set_error(NH_ERR_CAPACITY, "raster window overflow");
     if (steps) {
-        std::vector<uint32_t> offs(steps);
-        HIPCHK(hipMemcpy(offs.data(), d_rast_off.p, steps * 4, hipMemcpyDeviceToHost));
-        std::vector<uint32_t> ids(nid);
-        if (nid) HIPCHK(hipMemcpy(ids.data(), d_rast_ids.p, nid * 4, hipMemcpyDeviceToHost));
+        std::vector<uint64_t> rows;
+        TRY(read_u64(d_rast_bits, rows, (size_t)steps * words));
         if (h_rast_cnt.empty()) h_rast_step0 = rast_step0;
         for (uint64_t s = 0; s < steps; ++s) {
-            const uint64_t a = offs[s], b = s + 1 < steps ? offs[s + 1] : nid;
-            if (b < a || b > nid) return set_error(NH_ERR_HIP, "raster offsets inconsistent");
-            std::sort(ids.begin() + a, ids.begin() + b);
-            h_rast_cnt.push_back((uint32_t)(b - a));
+            const uint64_t *row = rows.data() + s * words;
+            uint32_t c = 0;
+            for (uint32_t w = 0; w < words; ++w) {
+                uint64_t m = row[w];
+                while (m) {
+                    h_rast_ids.push_back(w * 64u + (uint32_t)__builtin_ctzll(m));
+                    m &= m - 1;
+                    ++c;
+                }
+            }
+            h_rast_cnt.push_back(c);
         }
-        h_rast_ids.insert(h_rast_ids.end(), ids.begin(), ids.end());
     }
-    rast_id0 = total;
     rast_step0 = t;
-    const uint64_t nc[3] = {rast_id0, rast_step0, 0};
+    const uint64_t nc[3] = {0, rast_step0, 0};
     HIPCHK(hipMemcpy(d_rast_ctl.p, nc, sizeof(nc), hipMemcpyHostToDevice));
     return NH_OK;
 }

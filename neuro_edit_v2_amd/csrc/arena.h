@@ -14,11 +14,12 @@
 namespace nh {
 
 void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool push,
-                 bool packed, bool list, hipStream_t st);
+                 bool packed, hipStream_t st);
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st);
 void launch_watch(const StepArgs &a, hipStream_t st);
 void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
-void launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
+void launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t lane_bytes, hipStream_t st);
+void launch_empty(uint32_t blocks, hipStream_t st);
 
 int set_error(int code, const std::string &msg);
 
@@ -111,9 +112,8 @@ class Arena {
     uint64_t dev_bytes = 0;
 
     // raster (host side accumulation)
-    uint32_t rast_steps = 0;
-    uint64_t rast_cap_ids = 0;
-    uint64_t rast_id0 = 0, rast_step0 = 0;      // origin of the device window
+    uint32_t rast_steps = 0;                    // rows of the device raster window
+    uint64_t rast_step0 = 0;                    // origin of the device window
     std::vector<uint32_t> h_rast_cnt;           // per step counts (drained)
     std::vector<uint32_t> h_rast_ids;           // ids (ascending per step)
     uint64_t h_rast_step0 = 0;                  // first step held on the host
@@ -122,7 +122,7 @@ class Arena {
     DBuf d_t, d_nets, d_wave_net, d_x, d_x0, d_s, d_ring, d_ext, d_prow, d_ptab, d_p, d_mreg;
     DBuf d_stim_ptr, d_stim_input, d_stim_w, d_stim_win;
     DBuf d_spk, d_cnt, d_bits, d_pn, d_last, d_net_spikes;
-    DBuf d_rast_ids, d_rast_off, d_rast_ctl;
+    DBuf d_rast_bits, d_rast_ctl;
     DBuf d_src_ptr, d_syn, d_delay16, d_w, d_totals;
     DBuf d_watch, d_xrec;
     DBuf d_row_hi, d_wcnt;     // served arena: explicit row ends, window counters
@@ -146,7 +146,6 @@ class Arena {
     StepArgs args() const;
     int launch_chunk(uint64_t t0, uint32_t k);
     int check_args(const StepArgs &a) const;
-    bool list_on() const;
     // direct launches with an event pair around every step kernel
     int profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,
                 uint64_t &aux_n);

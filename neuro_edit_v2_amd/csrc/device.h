@@ -24,27 +24,24 @@ __device__ __forceinline__ uint64_t noise_key(uint32_t i, uint64_t t, uint64_t s
            seed * 0x94D049BB133111EBull;
 }
 
-// Poisson-kick decision: top 32 bits of splitmix64 over the (i, t, seed)
+// Poisson-kick decision: top 32 bits of h = splitmix64 over the (i, t, seed)
 // key, scaled by 2^-32, compared with rate*dt/1000
 // (sim/src/state.rs:283-301, sim/src/kernels.rs:69-77,128-130).
-__device__ __forceinline__ bool kick_fires(uint32_t i, uint64_t t, uint64_t seed, float p) {
-    uint32_t h = (uint32_t)(mix64(noise_key(i, t, seed)) >> 32);
-    return (float)h * 0x1p-32f < p;
+__device__ __forceinline__ bool kick_fires(uint64_t h, float p) {
+    return (float)(uint32_t)(h >> 32) * 0x1p-32f < p;
 }
 
-// Gaussian current draw: Irwin-Hall(12) over the twelve 16-bit fields of
-// three splitmix64 hashes of key + (k+1)*0xD1B54A32D192ED03; integer sum,
-// exact conversion, exact scale -- mean 0, variance 1 - 2^-32.
-__device__ __forceinline__ float gauss_draw(uint32_t i, uint64_t t, uint64_t seed) {
-    const uint64_t key = noise_key(i, t, seed);
-    uint32_t sum = 0;
-#pragma unroll
-    for (uint32_t k = 1; k <= 3; ++k) {
-        const uint64_t h = mix64(key + (uint64_t)k * 0xD1B54A32D192ED03ull);
-        const uint32_t lo = (uint32_t)h, hi = (uint32_t)(h >> 32);
-        sum += (lo & 0xFFFFu) + (lo >> 16) + (hi & 0xFFFFu) + (hi >> 16);
-    }
-    return (float)((int32_t)sum - 393210) * 0x1p-16f;
+// Gaussian current draw: Irwin-Hall(12) over twelve 8-bit fields -- the four
+// bytes of the low half of the kick hash h0 (the high half decides the kick)
+// and the eight bytes of h1 = splitmix64(key + 0xD1B54A32D192ED03); integer
+// byte sums (v_sad_u8 against zero), exact conversion, exact 2^-8 scale --
+// mean 0, variance 1 - 2^-16.
+__device__ __forceinline__ float gauss_draw(uint64_t h0, uint64_t key) {
+    const uint64_t h1 = mix64(key + 0xD1B54A32D192ED03ull);
+    uint32_t sum = __builtin_amdgcn_sad_u8((uint32_t)h0, 0u, 0u);
+    sum = __builtin_amdgcn_sad_u8((uint32_t)h1, 0u, sum);
+    sum = __builtin_amdgcn_sad_u8((uint32_t)(h1 >> 32), 0u, sum);
+    return (float)((int32_t)sum - 1530) * 0x1p-8f;
 }
 
 // Portable exp: Cody & Waite reduction with fdlibm's two-part ln2, degree-7

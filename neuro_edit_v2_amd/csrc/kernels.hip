@@ -27,15 +27,32 @@
 namespace nh {
 
 template <int RING> struct Ring;
+// Ring read scale: x / scale (sim/src/kernels.rs:60), or x * 2^-k when the
+// scale is 2^k with 0 <= k <= 64 -- both exact then, so the same bits.
+struct RingScale {
+    float scale, inv;
+    bool pow2;
+    __device__ __forceinline__ explicit RingScale(float s) : scale(s) {
+        const uint32_t b = __float_as_uint(s);
+        pow2 = (b & 0x7FFFFFu) == 0u && b >= 0x3F800000u && b <= 0x5F800000u;
+        inv = __uint_as_float(0x7F000000u - b);
+    }
+};
+
 template <> struct Ring<0> {
     using T = float;
-    static __device__ __forceinline__ float get(T v, float) { return v; }
+    static __device__ __forceinline__ float get(T v, const RingScale &) { return v; }
     static __device__ __forceinline__ void add(T *p, T v) { unsafeAtomicAdd(p, v); }
 };
 template <> struct Ring<1> {
     using T = int32_t;
     // sim/src/kernels.rs:60: (float)x / WSCALE
-    static __device__ __forceinline__ float get(T v, float scale) { return (float)v / scale; }
+    static __device__ __forceinline__ float get(T v, const RingScale &rs) {
+        if (rs.pow2) return (float)v * rs.inv;
+        // a real (uniform) branch: keeps the division off the common path
+        asm volatile("");
+        return (float)v / rs.scale;
+    }
     static __device__ __forceinline__ void add(T *p, T v) { atomicAdd(p, v); }
 };
 
@@ -48,27 +65,111 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Entries pushed per lane before the first atomic of a batch: every load of
+// a batch is in flight at once, so a batch costs one memory round trip.
+constexpr uint32_t kPushUnroll = 4;
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;  // batch slot without an entry
+
+// One ring delivery: synapse j of the delay-sorted rows (target and delay
+// packed in one word, or target plus a separate 16-bit delay).
+template <int RING, bool PACKED>
+__device__ __forceinline__ void deliver(const StepArgs &a, uint32_t sy, uint32_t d16,
+                                        typename Ring<RING>::T wv, uint32_t slot0) {
+    using RT = typename Ring<RING>::T;
+    uint32_t tgt, d;
+    if (PACKED) {
+        tgt = sy & 0x07FFFFFFu;
+        d = sy >> 27;
+    } else {
+        tgt = sy;
+        d = d16;
+    }
+    uint32_t sl = slot0 + d;
+    sl = sl >= a.D ? sl - a.D : sl;
+    Ring<RING>::add((RT *)a.ring + (size_t)sl * a.n_total + tgt, wv);
+}
+
+// Load the entries j[0..kPushUnroll) (kNoEntry = none) and deliver them:
+// all loads first, then the atomics.
+template <int RING, bool PACKED>
+__device__ __forceinline__ void deliver_batch(const StepArgs &a, const uint32_t (&j)[kPushUnroll],
+                                              uint32_t slot0) {
+    using RT = typename Ring<RING>::T;
+    const RT *w = (const RT *)a.w;
+    uint32_t sy[kPushUnroll], d16[kPushUnroll];
+    RT wv[kPushUnroll];
+#pragma unroll
+    for (uint32_t r = 0; r < kPushUnroll; ++r) {
+        if (j[r] != kNoEntry) {
+            sy[r] = a.syn[j[r]];
+            wv[r] = w[j[r]];
+            d16[r] = PACKED ? 0u : a.delay16[j[r]];
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kPushUnroll; ++r)
+        if (j[r] != kNoEntry) deliver<RING, PACKED>(a, sy[r], d16[r], wv[r], slot0);
+}
+
 // Push one spike's out-synapse row into the ring; lanes stride the row
-// (delay-sorted, so a run of lanes hits one ring row with ascending targets).
+// (delay-sorted, so a run of lanes hits one ring row with ascending targets),
+// kPushUnroll strides per batch.
 template <int RING, bool PACKED>
 __device__ __forceinline__ void push_row(const StepArgs &a, uint32_t lo, uint32_t hi,
                                          uint32_t slot0, uint32_t lane) {
-    using RT = typename Ring<RING>::T;
-    RT *ring = (RT *)a.ring;
-    const RT *w = (const RT *)a.w;
-    for (uint32_t j = lo + lane; j < hi; j += kWave) {
-        const uint32_t sy = a.syn[j];
-        uint32_t tgt, d;
-        if (PACKED) {
-            tgt = sy & 0x07FFFFFFu;
-            d = sy >> 27;
-        } else {
-            tgt = sy;
-            d = a.delay16[j];
+    for (uint32_t j0 = lo + lane; j0 < hi; j0 += kPushUnroll * kWave) {
+        uint32_t j[kPushUnroll];
+#pragma unroll
+        for (uint32_t r = 0; r < kPushUnroll; ++r) {
+            const uint32_t jr = j0 + r * kWave;
+            j[r] = jr < hi ? jr : kNoEntry;
         }
-        uint32_t sl = slot0 + d;
-        sl = sl >= a.D ? sl - a.D : sl;
-        Ring<RING>::add(ring + (size_t)sl * a.n_total + tgt, w[j]);
+        deliver_batch<RING, PACKED>(a, j, slot0);
+    }
+}
+
+// Push the rows of every spiking lane of a wave as one list: the rows are
+// concatenated in lane order and the wave walks the list kPushUnroll*64
+// entries at a time, so a wave whose spikes own few synapses pays one load
+// round trip in total instead of one per spike and per 64 entries.
+template <int RING, bool PACKED>
+__device__ __forceinline__ void push_wave(const StepArgs &a, uint64_t mask, uint32_t row_lo,
+                                          uint32_t row_hi, uint32_t slot0, uint32_t lane) {
+    constexpr uint32_t kSpan = kPushUnroll * kWave;
+    uint32_t total = 0;
+    for (uint64_t m = mask; m;) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        total += __builtin_amdgcn_readlane(row_hi, b) - __builtin_amdgcn_readlane(row_lo, b);
+    }
+    // `live`: spikes whose rows are not fully issued (always a suffix of
+    // the spikes in lane order); `live_start`: list offset of the first
+    uint64_t live = mask;
+    uint32_t live_start = 0;
+    for (uint32_t base = 0; base < total; base += kSpan) {
+        const uint32_t end = base + kSpan;
+        uint32_t j[kPushUnroll];
+#pragma unroll
+        for (uint32_t r = 0; r < kPushUnroll; ++r) j[r] = kNoEntry;
+        uint32_t s0 = live_start;
+        for (uint64_t m = live; m;) {
+            if (s0 >= end) break;
+            const uint32_t b = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t lo = __builtin_amdgcn_readlane(row_lo, b);
+            const uint32_t d = __builtin_amdgcn_readlane(row_hi, b) - lo;
+#pragma unroll
+            for (uint32_t r = 0; r < kPushUnroll; ++r) {
+                const uint32_t u = base + r * kWave + lane - s0;
+                if (u < d) j[r] = lo + u;
+            }
+            if (s0 + d <= end) {
+                live &= ~(1ull << b);
+                live_start = s0 + d;
+            }
+            s0 += d;
+        }
+        deliver_batch<RING, PACKED>(a, j, slot0);
     }
 }
 
@@ -91,8 +192,9 @@ __device__ __forceinline__ void push_custom(const StepArgs &a, uint32_t src, uin
 // ---------------------------------------------------------------- step
 // PMODE: 0 parameter rows in the kernel arguments (<= 4 rows), 1 per-neuron
 // rows, 2 global row table.  MMODE: 0 izh, 1 adex, 2 mixed.
-// LIST: keep the per-step spike list (two-kernel shape or raster on).
-template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LIST>
+// PUSH: the fused shape (own-spike push); otherwise the step appends its
+// spikes to the per-step list that k_scatter reads.
+template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     using RT = typename Ring<RING>::T;
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -100,8 +202,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     if (wave >= a.words) return;
     const uint32_t i = wave * kWave + lane;
     const uint32_t slot = a.slot;
-    const uint32_t net = a.single_net ? 0u : uniform((uint32_t)a.wave_net[wave]);
-    const NetRec &nr = a.single_net ? a.net0 : a.nets[net];
+    // The network record is copied by value, so its fields are read once,
+    // up front, into scalar registers (a reference that may point at either
+    // the kernel arguments or the table makes every field a separate vector
+    // load at its point of use).
+    NetRec nr;
+    if (a.single_net) {
+        nr = a.net0;
+    } else {
+        nr = a.nets[uniform((uint32_t)a.wave_net[wave])];
+    }
 
     // ---- first batch of independent loads
     RT *ring = (RT *)a.ring;
@@ -125,7 +235,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     bool spiked = false;
     if (act) {
         ring[ri] = (RT)0;
-        const float arr = Ring<RING>::get(rv, nr.int_scale);
+        const RingScale rsc(nr.int_scale);
+        const float arr = Ring<RING>::get(rv, rsc);
         const float sv = s_old + arr;
         float I = sv + ext;
         if (a.xt_ptr) {
@@ -137,7 +248,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
                 const uint32_t kx = a.xt_idx[j];
                 const RT av = xr[kx];
                 xr[kx] = (RT)0;
-                const float xv = a.xsyn[kx] + Ring<RING>::get(av, nr.int_scale);
+                const float xv = a.xsyn[kx] + Ring<RING>::get(av, rsc);
                 I += xv;
                 a.xsyn[kx] = xv * a.xdecay[kx];
             }
@@ -178,8 +289,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
         const bool is_adex = MMODE == 1 || (MMODE == 2 && model == 1);
         // sim/src/cpu.rs:240-245: kick * model gain (C for AdEx, adex.rs:189-191)
         const float gain = is_adex ? pr[0] : 1.0f;
-        if (nr.pkick > 0.0f && kick_fires(local, tl, nr.seed, nr.pkick)) I += nr.kick * gain;
-        if (nr.sd != 0.0f) I += (nr.sd * gauss_draw(local, tl, nr.seed)) * gain;
+        if (nr.pkick > 0.0f || nr.sd != 0.0f) {
+            // one hash serves the kick (high half) and the Gaussian (low half)
+            const uint64_t key = noise_key(local, tl, nr.seed);
+            const uint64_t h0 = mix64(key);
+            if (nr.pkick > 0.0f && kick_fires(h0, nr.pkick)) I += nr.kick * gain;
+            if (nr.sd != 0.0f) I += (nr.sd * gauss_draw(h0, key)) * gain;
+        }
         if (is_adex)
             spiked = adex_step(pr, xs, I, nr.dt);
         else
@@ -198,53 +314,40 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
 
     // ---- wave64 ballot: the mask IS this wave's word of the spike bitmap
     const uint64_t mask = __ballot(spiked);
-    if (a.bits && lane == 0) a.bits[(size_t)(t % a.bits_rows) * a.words + wave] = mask;
-    const uint32_t L = (uint32_t)(t % 3);
-    if (LIST && blockIdx.x == 0 && threadIdx.x == 0) {
-        // spike-list bookkeeping of step t (every value read here is final:
-        // written by earlier launches)
-        const uint32_t Lp = (L + 2) % 3, Ln = (L + 1) % 3;
-        const uint64_t base_t = a.totals[2 + Lp] + a.cnt[Lp];
-        a.totals[2 + L] = base_t;
-        a.cnt[Ln] = 0;
-        if (a.rast_off) {
+    if (lane == 0) {
+        if (a.bits) a.bits[(size_t)(t % a.bits_rows) * a.words + wave] = mask;
+        if (a.rast_bits) {
+            // raster window row of this step; the host drains the window
+            // before it fills, the flag reports a step outside it
             const uint64_t rs = t - a.rast_ctl[1];
-            if (rs < a.rast_cap_steps) a.rast_off[rs] = (uint32_t)(base_t - a.rast_ctl[0]);
+            if (rs < a.rast_cap_steps) a.rast_bits[(size_t)rs * a.words + wave] = mask;
+            else a.rast_ctl[2] = 1;
         }
     }
+    const uint32_t L = (uint32_t)(t % 3);
+    // two-kernel shape: the list of step t+1 starts empty (its appends come
+    // in the next launch, after this one has finished)
+    if (!PUSH && blockIdx.x == 0 && threadIdx.x == 0) a.cnt[(L + 1) % 3] = 0;
     if (!mask) return;
     // Spike and event totals are not counted here: they follow exactly from
     // pn_total and the out-degrees on readback, so no wave contends for a
     // shared counter.
-    if (LIST) {
+    if (!PUSH) {
         const uint32_t nsp = (uint32_t)__popcll(mask);
         uint32_t off = 0;
         if (lane == 0) off = atomicAdd(&a.cnt[L], nsp);
         off = uniform(off);
-        if (spiked) {
-            const uint32_t pos = off + wave_rank(mask);
-            a.spk[(size_t)L * a.n_total + pos] = i;
-            if (a.rast_ids) {
-                const uint32_t Lp = (L + 2) % 3;
-                const uint64_t base_t = a.totals[2 + Lp] + a.cnt[Lp];
-                const uint64_t rp = base_t + pos - a.rast_ctl[0];
-                if (rp < a.rast_cap_ids)
-                    a.rast_ids[rp] = i;
-                else
-                    a.rast_ctl[2] = 1;
-            }
-        }
+        if (spiked) a.spk[(size_t)L * a.n_total + off + wave_rank(mask)] = i;
     }
     if (PUSH) {
-        // push the rows of this wave's spikes, one spike at a time, all lanes
-        uint64_t m = mask;
-        while (m) {
-            const uint32_t b = (uint32_t)__builtin_ctzll(m);
-            m &= m - 1;
-            const uint32_t lo = __builtin_amdgcn_readlane(row_lo, b);
-            const uint32_t hi = __builtin_amdgcn_readlane(row_hi, b);
-            push_row<RING, PACKED>(a, lo, hi, slot, lane);
-            if (a.xs_lo) push_custom<RING>(a, wave * kWave + b, slot, lane);
+        push_wave<RING, PACKED>(a, mask, row_lo, row_hi, slot, lane);
+        if (a.xs_lo) {
+            // custom-tau rows (rare): one spike at a time
+            for (uint64_t m = mask; m;) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                push_custom<RING>(a, wave * kWave + b, slot, lane);
+            }
         }
     }
 }
@@ -296,66 +399,70 @@ void launch_clear_range(const StepArgs &a, uint32_t base, uint32_t len, hipStrea
     hipLaunchKernelGGL(k_clear_range, dim3((len + 255) / 256), dim3(256), 0, st, a, base, len);
 }
 
-// Plain copy of n16 16-byte words (byte-counter calibration).
-__global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16) {
+// Plain copy of n words of T (byte-counter calibration; T = uint32_t gives
+// the step kernel's 4-byte lanes, uint4 the widest loads).
+template <typename T>
+__global__ void k_copy(const T *__restrict__ src, T *__restrict__ dst, uint64_t n) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         dst[i] = src[i];
 }
 
-void launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st) {
-    const uint64_t n16 = bytes / 16;
-    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16);
+void launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t lane_bytes, hipStream_t st) {
+    if (lane_bytes == 16)
+        hipLaunchKernelGGL(k_copy<uint4>, dim3(4096), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst,
+                           bytes / 16);
+    else
+        hipLaunchKernelGGL(k_copy<uint32_t>, dim3(8192), dim3(256), 0, st, (const uint32_t *)src,
+                           (uint32_t *)dst, bytes / 4);
+}
+
+// Empty kernel: the dispatch cost of one graph node (launch-floor probe).
+__global__ void k_empty() {}
+
+void launch_empty(uint32_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_empty, dim3(blocks), dim3(256), 0, st);
 }
 
 // End of a chunk: advance the device step base.
 __global__ void k_advance(uint64_t *t_dev, uint32_t k) { *t_dev += k; }
 
 // ------------------------------------------------------------ dispatch
-template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LIST>
+template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED>
 static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LIST>), dim3(blocks),
-                       dim3(a.block), 0, st, a);
+    hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED>), dim3(blocks), dim3(a.block), 0, st, a);
 }
 
 template <int RING, int PMODE, int MMODE>
-static void launch_m(const StepArgs &a, uint32_t blocks, bool push, bool packed, bool list,
-                     hipStream_t st) {
-    if (!push) {
-        go<RING, PMODE, MMODE, false, false, true>(a, blocks, st);
-    } else if (packed) {
-        if (list) go<RING, PMODE, MMODE, true, true, true>(a, blocks, st);
-        else go<RING, PMODE, MMODE, true, true, false>(a, blocks, st);
-    } else {
-        if (list) go<RING, PMODE, MMODE, true, false, true>(a, blocks, st);
-        else go<RING, PMODE, MMODE, true, false, false>(a, blocks, st);
-    }
+static void launch_m(const StepArgs &a, uint32_t blocks, bool push, bool packed, hipStream_t st) {
+    if (!push) go<RING, PMODE, MMODE, false, false>(a, blocks, st);
+    else if (packed) go<RING, PMODE, MMODE, true, true>(a, blocks, st);
+    else go<RING, PMODE, MMODE, true, false>(a, blocks, st);
 }
 
 template <int RING, int PMODE>
-static void launch_p(const StepArgs &a, uint32_t blocks, int mmode, bool push, bool packed, bool list,
-                     hipStream_t st) {
+static void launch_p(const StepArgs &a, uint32_t blocks, int mmode, bool push, bool packed, hipStream_t st) {
     switch (mmode) {
-    case 0: launch_m<RING, PMODE, 0>(a, blocks, push, packed, list, st); break;
-    case 1: launch_m<RING, PMODE, 1>(a, blocks, push, packed, list, st); break;
-    default: launch_m<RING, PMODE, 2>(a, blocks, push, packed, list, st); break;
+    case 0: launch_m<RING, PMODE, 0>(a, blocks, push, packed, st); break;
+    case 1: launch_m<RING, PMODE, 1>(a, blocks, push, packed, st); break;
+    default: launch_m<RING, PMODE, 2>(a, blocks, push, packed, st); break;
     }
 }
 
 template <int RING>
 static void launch_r(const StepArgs &a, uint32_t blocks, int pmode, int mmode, bool push, bool packed,
-                     bool list, hipStream_t st) {
+                     hipStream_t st) {
     switch (pmode) {
-    case 0: launch_p<RING, 0>(a, blocks, mmode, push, packed, list, st); break;
-    case 1: launch_p<RING, 1>(a, blocks, mmode, push, packed, list, st); break;
-    default: launch_p<RING, 2>(a, blocks, mmode, push, packed, list, st); break;
+    case 0: launch_p<RING, 0>(a, blocks, mmode, push, packed, st); break;
+    case 1: launch_p<RING, 1>(a, blocks, mmode, push, packed, st); break;
+    default: launch_p<RING, 2>(a, blocks, mmode, push, packed, st); break;
     }
 }
 
 void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool push,
-                 bool packed, bool list, hipStream_t st) {
-    if (ring == 0) launch_r<0>(a, blocks, pmode, mmode, push, packed, list, st);
-    else launch_r<1>(a, blocks, pmode, mmode, push, packed, list, st);
+                 bool packed, hipStream_t st) {
+    if (ring == 0) launch_r<0>(a, blocks, pmode, mmode, push, packed, st);
+    else launch_r<1>(a, blocks, pmode, mmode, push, packed, st);
 }
 
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st) {

@@ -89,11 +89,9 @@ struct StepArgs {
     uint32_t *last_spike;        // per-neuron last spike step (kNoSpike = never)
     uint64_t *net_spikes;        // per-network spike totals or null
     // raster stream
-    uint32_t *rast_ids;
-    uint32_t *rast_off;          // per step: ids before it in the window
-    uint64_t *rast_ctl;          // [0] id origin, [1] step origin, [2] overflow
-    uint64_t rast_cap_ids;
-    uint32_t rast_cap_steps;
+    uint64_t *rast_bits;         // raster window: [step - origin][words] spike words, or null
+    uint64_t *rast_ctl;          // [1] step origin of the window, [2] overflow flag
+    uint32_t rast_cap_steps;     // rows of rast_bits
     // windowed spike counters [window][n_total] (served form) or null
     uint32_t *wcnt;
     uint32_t n_win;
@@ -105,7 +103,7 @@ struct StepArgs {
     const uint32_t *syn;         // packed (tgt | delay << 27) or tgt
     const uint16_t *delay16;     // unpacked delays (null when packed)
     const void *w;               // float or int32 per synapse
-    uint64_t *totals;            // [0] spikes, [1] events, [2+L] list fill before step of list L
+    uint64_t *totals;            // spare counters (cleared with the arena)
     // custom-tau synapses (own exponential trace, sim/src/state.rs:230-274):
     // per slot k a trace xsyn[k] with decay xdecay[k] and an arrival ring
     // xring[slot][k]; target-major lists xt for the fold-in, per-source

@@ -211,8 +211,9 @@ int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_c
     return NH_OK;
 }
 
-int nh_probe_copy(uint64_t bytes, uint32_t reps, double *ms) {
-    if (!ms || bytes < 16 || reps == 0) return set_error(NH_ERR_ARG, "bad probe arguments");
+int nh_probe_copy(uint64_t bytes, uint32_t reps, uint32_t lane_bytes, double *ms) {
+    if (!ms || bytes < 16 || reps == 0 || (lane_bytes != 4 && lane_bytes != 16))
+        return set_error(NH_ERR_ARG, "bad probe arguments");
     if (!nh_available()) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
     bytes &= ~uint64_t(15);
     nh::DBuf a, b;
@@ -223,9 +224,9 @@ int nh_probe_copy(uint64_t bytes, uint32_t reps, double *ms) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    nh::launch_copy(a.p, b.p, bytes, st);  // warm
+    nh::launch_copy(a.p, b.p, bytes, lane_bytes, st);  // warm
     HIPCHK(hipEventRecord(e0, st));
-    for (uint32_t r = 0; r < reps; ++r) nh::launch_copy(a.p, b.p, bytes, st);
+    for (uint32_t r = 0; r < reps; ++r) nh::launch_copy(a.p, b.p, bytes, lane_bytes, st);
     HIPCHK(hipEventRecord(e1, st));
     HIPCHK(hipStreamSynchronize(st));
     float t = 0.f;
@@ -233,6 +234,38 @@ int nh_probe_copy(uint64_t bytes, uint32_t reps, double *ms) {
     *ms = t / reps;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    return NH_OK;
+}
+
+int nh_probe_launch(uint32_t blocks, uint32_t launches, double *us) {
+    if (!us || blocks == 0 || launches == 0 || launches > 100000)
+        return set_error(NH_ERR_ARG, "bad probe arguments");
+    if (!nh_available()) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipGraph_t g;
+    hipGraphExec_t ge;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (uint32_t k = 0; k < launches; ++k) nh::launch_empty(blocks, st);
+    HIPCHK(hipStreamEndCapture(st, &g));
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipGraphLaunch(ge, st));  // warm
+    HIPCHK(hipEventRecord(e0, st));
+    const int reps = 5;
+    for (int r = 0; r < reps; ++r) HIPCHK(hipGraphLaunch(ge, st));
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *us = (double)t * 1e3 / ((double)reps * launches);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
     (void)hipStreamDestroy(st);
     return NH_OK;
 }

@@ -38,23 +38,29 @@ static uint64_t mix64(uint64_t h)
 }
 
 /* Gaussian background draw (not in the reference; defined here and in the
- * product identically).  Three splitmix64 hashes over the same (i, t, seed)
- * tuple as the kick hash plus (k+1) * 0xD1B54A32D192ED03 give 12 16-bit
- * fields u_0..u_11; g = (sum(u) - 393210) / 65536 (Irwin-Hall(12) on a
- * 16-bit lattice: mean exactly 0, variance 1 - 2^-32, range +-6).  Only
+ * product identically).  Irwin-Hall(12) over twelve 8-bit fields: the four
+ * bytes of the LOW half of the kick hash h0 = splitmix64(key) (its high half
+ * decides the kick) and the eight bytes of h1 = splitmix64(key +
+ * 0xD1B54A32D192ED03), where key is the kick key of (i, t, seed).
+ * g = (sum(bytes) - 1530) / 256: mean exactly 0, variance (12 * (256^2 - 1)
+ * / 12) / 65536 = 1 - 2^-16, range +-5.98, values on a 2^-8 lattice.  Only
  * integer work, one exact int->float conversion and an exact power-of-two
- * scale, so every IEEE implementation returns the same bits. */
+ * scale, so every IEEE implementation returns the same bits; two hashes per
+ * neuron-step in all (one shared with the kick). */
+static uint32_t byte_sum32(uint32_t x)
+{
+    return (x & 0xFF) + ((x >> 8) & 0xFF) + ((x >> 16) & 0xFF) + (x >> 24);
+}
+
 float or_gauss_draw(uint32_t i, uint64_t t, uint64_t seed)
 {
-    uint64_t base = (uint64_t)i * 0x9E3779B97F4A7C15ull + t * 0xBF58476D1CE4E5B9ull +
-                    seed * 0x94D049BB133111EBull;
-    uint32_t sum = 0;
-    for (uint64_t k = 0; k < 3; ++k) {
-        uint64_t h = mix64(base + (k + 1) * 0xD1B54A32D192ED03ull);
-        sum += (uint32_t)(h & 0xFFFF) + (uint32_t)((h >> 16) & 0xFFFF) +
-               (uint32_t)((h >> 32) & 0xFFFF) + (uint32_t)(h >> 48);
-    }
-    return (float)((int32_t)sum - 393210) * (1.0f / 65536.0f);
+    uint64_t key = (uint64_t)i * 0x9E3779B97F4A7C15ull + t * 0xBF58476D1CE4E5B9ull +
+                   seed * 0x94D049BB133111EBull;
+    uint64_t h0 = mix64(key);
+    uint64_t h1 = mix64(key + 0xD1B54A32D192ED03ull);
+    uint32_t sum = byte_sum32((uint32_t)h0) + byte_sum32((uint32_t)h1) +
+                   byte_sum32((uint32_t)(h1 >> 32));
+    return (float)((int32_t)sum - 1530) * (1.0f / 256.0f);
 }
 
 /* Portable single-precision exp for AdEx (not libm's): Cody & Waite (1980)

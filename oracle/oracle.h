@@ -26,8 +26,8 @@
  * product so both sides can be compared bit for bit):
  *   - int ring on the CPU (the reference has it on CUDA only) with the scale
  *     as a parameter (4096 in the reference, 65536 = Q16.16 also used);
- *   - Gaussian background current "noise_sd" (Irwin-Hall over 12 x 16-bit
- *     fields of three splitmix64 hashes);
+ *   - Gaussian background current "noise_sd" (Irwin-Hall over 12 x 8-bit
+ *     fields: the low half of the kick hash and one more splitmix64 hash);
  *   - exp_mode 1: a portable exp (Cody-Waite reduction + degree-7 Taylor
  *     polynomial, plain f32 operations) for AdEx, so CPU and GPU agree
  *     exactly; exp_mode 0 calls libm expf as the reference CPU backend does;

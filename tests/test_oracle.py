@@ -93,14 +93,14 @@ def test_noise_streams_are_uncorrelated():
 
 
 def test_gauss_draw_moments_and_lattice():
-    """Irwin-Hall(12) on a 2^-16 lattice: mean 0, variance 1 - 2^-32,
-    support within +-6, every value an exact multiple of 2^-16; neighbouring
-    (i, t, seed) draws uncorrelated; the draw is independent of the kick hash."""
+    """Irwin-Hall(12) of 8-bit fields on a 2^-8 lattice: mean 0, variance
+    1 - 2^-16, support within +-6, every value an exact multiple of 2^-8;
+    neighbouring (i, t, seed) draws uncorrelated."""
     g = np.array([O.gauss_draw(i, t, 11) for i in range(200) for t in range(200)], np.float64)
     assert abs(g.mean()) < 0.02
     assert abs(g.std() - 1.0) < 0.02
     assert np.all(np.abs(g) <= 6.0)
-    assert np.all((g * 65536.0) == np.round(g * 65536.0))
+    assert np.all((g * 256.0) == np.round(g * 256.0))
     # kurtosis of Irwin-Hall(12) is 3 - 1.2/12 = 2.9
     k = ((g - g.mean()) ** 4).mean() / g.var() ** 2
     assert 2.8 < k < 3.0
@@ -112,17 +112,31 @@ def test_gauss_draw_moments_and_lattice():
         assert abs(np.corrcoef(a, other)[0, 1]) < 0.08
 
 
-def test_gauss_draw_sum_of_fields_closed_form():
-    """Recompute one draw from splitmix64 by the definition (three hashes of
-    key + (k+1)*0xD1B54A32D192ED03, four 16-bit fields each)."""
+def test_gauss_draw_independent_of_the_kick_decision():
+    """The draw shares the kick hash but only its low half: it must not
+    correlate with the high half that decides the kick (were the fields
+    taken from the high half, the correlation would be about 0.5)."""
     M = (1 << 64) - 1
-    for (i, t, s) in [(0, 0, 0), (3, 17, 5), (99999, 123456, 42)]:
+    g, top = [], []
+    for i in range(300):
+        for t in range(100):
+            key = (i * 0x9E3779B97F4A7C15 + t * 0xBF58476D1CE4E5B9 + 3 * 0x94D049BB133111EB) & M
+            top.append(splitmix64(key) >> 32)
+            g.append(O.gauss_draw(i, t, 3))
+    assert abs(np.corrcoef(np.array(g), np.array(top, np.float64))[0, 1]) < 0.03
+
+
+def test_gauss_draw_sum_of_fields_closed_form():
+    """Recompute draws from splitmix64 by the definition: bytes of the low
+    half of splitmix64(key) plus the 8 bytes of splitmix64(key +
+    0xD1B54A32D192ED03), minus 1530, over 256."""
+    M = (1 << 64) - 1
+    for (i, t, s) in [(0, 0, 0), (3, 17, 5), (99999, 123456, 42), (2**32 - 1, 2**40, 2**63)]:
         key = (i * 0x9E3779B97F4A7C15 + t * 0xBF58476D1CE4E5B9 + s * 0x94D049BB133111EB) & M
-        tot = 0
-        for k in range(3):
-            h = splitmix64((key + (k + 1) * 0xD1B54A32D192ED03) & M)
-            tot += sum((h >> (16 * q)) & 0xFFFF for q in range(4))
-        assert O.gauss_draw(i, t, s) == (tot - 393210) / 65536.0
+        h0 = splitmix64(key)
+        h1 = splitmix64((key + 0xD1B54A32D192ED03) & M)
+        tot = sum((h0 >> (8 * q)) & 0xFF for q in range(4)) + sum((h1 >> (8 * q)) & 0xFF for q in range(8))
+        assert O.gauss_draw(i, t, s) == (tot - 1530) / 256.0
 
 
 def test_portable_exp_accuracy():
