@@ -195,6 +195,7 @@ def bench_large(args, kw):
     kbytes = n * B_NEURON + ev_step * B_EVENT
     kavg_ms = gpu_ms / steps
     achieved = kbytes / (kavg_ms * 1e-3) / 1e9
+    traffic = load_traffic(f"traffic_fused_cube{n // 1000}k_{'int' if kw['int_weights'] else 'float'}.txt")
     return {"workload": f"random-cube Izhikevich {n} neurons x Poisson(100) out-synapses, "
                         f"dt 0.5 ms, {steps} steps, same noise and ring",
             "neurons": n, "synapses": n_syn, "gen_s": round(t_gen, 2),
@@ -207,6 +208,7 @@ def bench_large(args, kw):
             "rate_hz": round(spikes / n / (steps * dt / 1000.0), 3),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": traffic.get("bytes_per_launch") if traffic else None,
                          "avg_launch_us": round(kavg_ms * 1e3, 3),
                          "algorithmic_bytes_per_launch": int(kbytes)}}
 
@@ -267,7 +269,10 @@ def main():
     dt = 0.5
     kw = dict(dt_ms=dt, synapse_tau_ms=5.0, noise_sd=1.0, noise_rate_hz=1.0, noise_kick_mv=20.0,
               seed=1, int_weights=args.ring == "int", int_scale=65536.0)
+    t_build = time.perf_counter()
     sim = Sim(net, SimConfig(exec_mode=args.exec_mode, step_shape=args.shape, **kw))
+    sim.synchronize()
+    t_build = time.perf_counter() - t_build      # host CSR -> HBM upload and set-up
     sim.run_steps(args.warmup)
     sim.synchronize()
     ev0, sp0 = sim.events_total(), sim.spikes_total()
@@ -373,6 +378,9 @@ def main():
         "events": int(events),
         "rate_hz": round(spikes / args.n / (args.steps * dt / 1000.0), 3),
         "gpu_event_ms": round(gpu_ms, 3),
+        # the network arrives as host arrays: the rate with its upload counted
+        "build_s": round(t_build, 4),
+        "events_per_s_incl_build": round(events / (wall + t_build), 1),
         "roofline": roof,
         "cpu_baseline": cpu,
         "large": large,
