@@ -4,6 +4,7 @@ import math
 
 import numpy as np
 
+from neuro_edit_v2_amd import _lib as L
 from neuro_edit_v2_amd import net as N
 from oracle import oracle as O
 
@@ -107,3 +108,45 @@ def test_grid_net_connection_probability():
     n_int = int(sum(1 for i in range(n) if all(0 < c < 11 for c in xyz(i))))
     freq = np.sum(interior & (d2 == 1)) / (6 * n_int)
     assert abs(freq - 0.3 * math.exp(-0.25)) < 0.02
+
+
+def test_rng_stream_window_draw_order_and_marginals():
+    """RngStream::window (sim/src/script.rs:2867-2894): n_src Bernoulli(p)
+    draws per step from the seeded Rng, then per listed pair one draw
+    against `strength` and, when it passes, one shared bit.  Checked against
+    the same draw order on the oracle's Rng (itself pinned by the published
+    SplitMix64 sequence), including an out-of-range pair that draws
+    nothing; then the marginal rate and the pair sharing at strength 1."""
+    from neuro_edit_v2_amd import workloads as W
+    from oracle import oracle as O
+    n_src, p, steps = 8, 0.3, 200
+    pairs = [(0, 1), (9, 2), (4, 6)]
+    h = W.rng_stream(77, n_src, p)
+    try:
+        quiet = W.stream_window(h, n_src, steps)
+        ev = W.stream_window(h, n_src, steps, pairs, 0.5)
+    finally:
+        L.lib().nh_rng_stream_free(h)
+    r = O.Rng(77)
+    want_q = np.array([[1.0 if r.f64() < np.float64(np.float32(p)) else 0.0 for _ in range(n_src)]
+                       for _ in range(steps)], np.float32)
+    want_e = np.zeros((steps, n_src), np.float32)
+    for t in range(steps):
+        for k in range(n_src):
+            want_e[t, k] = 1.0 if r.f64() < np.float64(np.float32(p)) else 0.0
+        for i, j in pairs:
+            if i >= n_src or j >= n_src:
+                continue
+            if r.f64() < np.float64(np.float32(0.5)):
+                bit = 1.0 if r.f64() < np.float64(np.float32(p)) else 0.0
+                want_e[t, i] = want_e[t, j] = bit
+    assert np.array_equal(quiet, want_q)
+    assert np.array_equal(ev, want_e)
+    h = W.rng_stream(5, n_src, 0.05)
+    try:
+        big = W.stream_window(h, n_src, 20000, [(2, 3)], 1.0)
+    finally:
+        L.lib().nh_rng_stream_free(h)
+    assert np.array_equal(big[:, 2], big[:, 3])
+    assert abs(big[:, [0, 1, 4, 5, 6, 7]].mean() - 0.05) < 0.004
+    assert abs(big[:, 2].mean() - 0.05) < 0.008
