@@ -229,6 +229,30 @@ void nh_rng_stream_free(nh_rng_stream *s);
 int nh_rng_stream_window(nh_rng_stream *s, uint32_t steps, int event, const uint32_t *pairs,
                          uint32_t n_pairs, float strength, float *out);
 
+/* .ne2 network files (net-format/src/io.rs:437-468 `io::load`): magic,
+ * version 1..7, bincode `Network`, older layouts lifted as the reference's
+ * loaders do (io.rs:14-330), validated (network.rs:1007-1150), then turned
+ * into the simulator layout with NeuronParams::from_network
+ * (sim/src/state.rs:98-153), synapse_csr(dt) (network.rs:654-692) and
+ * StimMap::from_network (sim/src/stim.rs:34-57).  dt_ms <= 0 takes the
+ * file's meta.dt_ms.  The caller reads the file; `info` (optional) gets the
+ * file's meta and table sizes. */
+typedef struct {
+    uint32_t version;
+    float dt_ms, conduction_speed, synapse_tau_ms, readout_tau_ms, noise_rate_hz, noise_kick_mv;
+    uint64_t seed;
+    uint32_t n_types, n_groups, n_inputs, n_samples, n_readouts, n_scripts;
+} nh_ne2_info;
+int nh_net_load_ne2(const uint8_t *bytes, uint64_t len, float dt_ms, nh_net **out, nh_ne2_info *info);
+/* Tables of a loaded file that callers address by index: type names (the
+ * lifted type table for v1-v3 files, io.rs:98-170), each neuron's type,
+ * input names (stimulus input k), groups (grid watches take members). */
+int nh_net_type_name(const nh_net *net, uint32_t k, char *buf, uint32_t cap);
+int nh_net_neuron_types(const nh_net *net, uint16_t *ty, uint32_t n);
+int nh_net_input_name(const nh_net *net, uint32_t k, char *buf, uint32_t cap);
+int nh_net_group(const nh_net *net, uint32_t k, char *name, uint32_t name_cap, uint32_t *members,
+                 uint32_t cap, uint32_t *n_members);
+
 /* ---------------- batched form ----------------
  * Many independent networks packed by offset tables into one launch
  * sequence; each network keeps its own config (dt must match), seed, t

@@ -40,6 +40,16 @@ class Config(C.Structure):
     ]
 
 
+class Ne2Info(C.Structure):
+    _fields_ = [
+        ("version", C.c_uint32), ("dt_ms", C.c_float), ("conduction_speed", C.c_float),
+        ("synapse_tau_ms", C.c_float), ("readout_tau_ms", C.c_float),
+        ("noise_rate_hz", C.c_float), ("noise_kick_mv", C.c_float), ("seed", C.c_uint64),
+        ("n_types", C.c_uint32), ("n_groups", C.c_uint32), ("n_inputs", C.c_uint32),
+        ("n_samples", C.c_uint32), ("n_readouts", C.c_uint32), ("n_scripts", C.c_uint32),
+    ]
+
+
 class NHError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[nh status {code}] {msg}")
@@ -106,6 +116,11 @@ def lib():
         "nh_batch_autotune": (i32, [vp, u32, C.c_char_p, u32]),
         "nh_net_free": (None, [vp]),
         "nh_net_view": (i32, [vp, P(NetDesc)]),
+        "nh_net_load_ne2": (i32, [C.c_char_p, u64, C.c_float, P(vp), P(Ne2Info)]),
+        "nh_net_type_name": (i32, [vp, u32, C.c_char_p, u32]),
+        "nh_net_neuron_types": (i32, [vp, vp, u32]),
+        "nh_net_input_name": (i32, [vp, u32, C.c_char_p, u32]),
+        "nh_net_group": (i32, [vp, u32, C.c_char_p, u32, vp, u32, P(u32)]),
         "nh_net_bench": (i32, [u32, u32, u64, f32, f32, f32, P(vp)]),
         "nh_net_cube": (i32, [u32, f64, u64, f32, P(vp)]),
         "nh_net_grid": (i32, [u32, u32, u32, f64, f64, f64, f32, f32, C.c_int, u32, u32, f32,

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/neuro_hip.h"
+#include "netgen.h"
 
 namespace nh {
 int set_error(int code, const std::string &msg);
@@ -59,18 +60,6 @@ const float kNonAdap[10] = {281.0f, 30.0f, -70.6f, -50.4f, 2.0f, 0.0f, 144.0f, 0
 
 }  // namespace
 
-struct nh_net {
-    uint32_t n = 0;
-    std::vector<uint32_t> src_ptr, tgt;
-    std::vector<float> w;
-    std::vector<uint16_t> delay;
-    std::vector<uint8_t> model_ids, model;
-    uint32_t n_params = 4, n_state = 2;
-    std::vector<float> p, x0;
-    uint32_t n_inputs = 0;
-    std::vector<uint32_t> stim_ptr, stim_input;
-    std::vector<float> stim_w;
-};
 
 // Per-neuron rows for a one- or two-kind net: izh init x = [-65, b*-65]
 // (izh.rs:300-303), adex x = [EL, 0] (adex.rs:149-152).
@@ -108,7 +97,7 @@ int nh_net_view(const nh_net *net, nh_net_desc *d) {
     d->tgt = net->tgt.data();
     d->w = net->w.data();
     d->delay = net->delay.data();
-    d->tau = nullptr;
+    d->tau = net->tau.empty() ? nullptr : net->tau.data();
     d->n_models = (uint32_t)net->model_ids.size();
     d->model_ids = net->model_ids.data();
     d->model = net->model.data();

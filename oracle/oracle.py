@@ -309,7 +309,8 @@ class OracleSim:
         self._keep = {
             "src_ptr": _arr(net["src_ptr"], np.uint32), "tgt": _arr(net["tgt"], np.uint32),
             "w": _arr(net["w"], np.float32), "delay": _arr(net["delay"], np.uint16),
-            "tau": _arr(net.get("tau", np.zeros(len(net["tgt"]))), np.float32),
+            "tau": _arr(net["tau"] if net.get("tau") is not None else np.zeros(len(net["tgt"])),
+                        np.float32),
             "model_ids": _arr(net["model_ids"], np.uint8), "model": _arr(net["model"], np.uint8),
             "p": _arr(net["p"], np.float32), "x0": _arr(net["x0"], np.float32),
             "stim_ptr": _arr(net.get("stim_ptr", np.zeros(self.n + 1)), np.uint32),
