@@ -93,6 +93,14 @@ typedef struct nh_sim nh_sim;
 
 /* Sim::with_params(net, cfg, BackendKind::Cuda, params) -- sim/src/lib.rs:217-233,
  * CudaSim::with_params -- sim/src/cuda.rs:130-239 */
+/* Host-only checks of a network and config, made by nh_create (and the
+ * batched/served entry points) before any device call: dt range
+ * (sim/src/lib.rs:35-39), the row-table consistency of NeuronParams::check
+ * (sim/src/state.rs:197-217), and the per-synapse / per-input checks of
+ * Network::validate (net-format/src/network.rs:1074-1150) on the flat
+ * arrays (targets in range, delays >= 1, finite weights and taus, int ring
+ * range).  NH_OK or NH_ERR_ARG with the reason in nh_last_error. */
+int nh_check_net(const nh_net_desc *net, const nh_config *cfg);
 int nh_create(const nh_net_desc *net, const nh_config *cfg, nh_sim **out);
 void nh_destroy(nh_sim *s);
 

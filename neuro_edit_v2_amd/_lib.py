@@ -74,6 +74,7 @@ def lib():
         "nh_config_default": (None, [P(Config)]),
         "nh_last_error": (C.c_char_p, []),
         "nh_available": (i32, []),
+        "nh_check_net": (i32, [P(NetDesc), P(Config)]),
         "nh_create": (i32, [P(NetDesc), P(Config), P(vp)]),
         "nh_destroy": (None, [vp]),
         "nh_run_steps": (i32, [vp, u64]),

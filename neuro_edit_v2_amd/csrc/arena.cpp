@@ -68,6 +68,75 @@ static uint32_t align64(uint64_t v) { return (uint32_t)((v + 63) & ~uint64_t(63)
 // sim/src/kernels.rs:61 -- (int)lrintf(w * WSCALE), round half to even.
 static int32_t quantize(float w, float scale) { return (int32_t)std::lrintf(w * scale); }
 
+// Host-side checks of one network and its configuration: everything the
+// device path assumes about the arrays (NeuronParams::check,
+// sim/src/state.rs:197-217; the per-synapse and per-input checks of
+// Network::validate, net-format/src/network.rs:1074-1150, on the flat
+// layout), made before any device call.  *max_delay gets the largest delay.
+int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay) {
+    auto fin = [](float x) { return std::isfinite(x); };
+    if (!(c.dt_ms > 0.0f) || !fin(c.dt_ms)) return set_error(NH_ERR_ARG, "dt_ms must be positive");
+    if (!(fin(c.synapse_tau_ms) && c.synapse_tau_ms >= 0.0f && fin(c.readout_tau_ms) &&
+          c.readout_tau_ms >= 0.0f))
+        return set_error(NH_ERR_ARG, "synapse and readout tau must be finite and >= 0");
+    if (!(fin(c.noise_rate_hz) && c.noise_rate_hz >= 0.0f && fin(c.noise_kick_mv) && fin(c.noise_sd) &&
+          c.noise_sd >= 0.0f))
+        return set_error(NH_ERR_ARG, "noise rate, kick and sd must be finite (rate, sd >= 0)");
+    if (c.int_weights && !(fin(c.int_scale) && c.int_scale > 0.0f))
+        return set_error(NH_ERR_ARG, "int_scale must be positive and finite");
+    if (d.n && (!d.src_ptr || !d.model || !d.p || !d.x0 || !d.model_ids))
+        return set_error(NH_ERR_ARG, "network arrays missing");
+    if (d.n_syn && (!d.tgt || !d.w || !d.delay)) return set_error(NH_ERR_ARG, "synapse arrays missing");
+    if (d.n_models == 0 && d.n) return set_error(NH_ERR_ARG, "empty model table");
+    if (d.n_models > 8) return set_error(NH_ERR_ARG, "model table too large");
+    if (d.n && d.n_state != 2) return set_error(NH_ERR_ARG, "state row width must be 2");
+    if (d.n_params > (uint32_t)kMaxParams) return set_error(NH_ERR_ARG, "param row too wide");
+    if (d.n > 0x7FFFFFC0u) return set_error(NH_ERR_CAPACITY, "network exceeds 2^31 neurons");
+    for (uint32_t m = 0; m < d.n_models; ++m)
+        if (d.model_ids[m] > 1) return set_error(NH_ERR_ARG, "unknown model id " + std::to_string(d.model_ids[m]));
+    for (uint32_t i = 0; i < d.n; ++i) {
+        if (d.model[i] >= d.n_models)
+            return set_error(NH_ERR_ARG, "neuron " + std::to_string(i) + " names model " +
+                                             std::to_string(d.model[i]) + " outside the model table");
+        const uint8_t id = d.model_ids[d.model[i]];
+        if (id == 1 && d.n_params < 10) return set_error(NH_ERR_ARG, "adex rows need 10 params");
+        if (id == 0 && d.n_params < 4) return set_error(NH_ERR_ARG, "izh rows need 4 params");
+    }
+    for (uint64_t q = 0; q < (uint64_t)d.n * d.n_params; ++q)
+        if (!fin(d.p[q])) return set_error(NH_ERR_ARG, "non-finite parameter");
+    for (uint64_t q = 0; q < (uint64_t)d.n * 2; ++q)
+        if (!fin(d.x0[q])) return set_error(NH_ERR_ARG, "non-finite initial state");
+    if (d.n && d.src_ptr[0] != 0) return set_error(NH_ERR_ARG, "src_ptr[0] must be 0");
+    for (uint32_t i = 0; i < d.n; ++i)
+        if (d.src_ptr[i + 1] < d.src_ptr[i]) return set_error(NH_ERR_ARG, "src_ptr not monotone");
+    const uint64_t E = d.n ? d.src_ptr[d.n] : 0;
+    if (E != d.n_syn) return set_error(NH_ERR_ARG, "src_ptr[n] != n_syn");
+    uint32_t maxd = 1;
+    const double lim = 2147483647.0;
+    for (uint64_t j = 0; j < E; ++j) {
+        if (d.tgt[j] >= d.n) return set_error(NH_ERR_ARG, "synapse target out of range");
+        if (d.delay[j] < 1) return set_error(NH_ERR_ARG, "delay must be >= 1 step");
+        if (!fin(d.w[j])) return set_error(NH_ERR_ARG, "non-finite synapse weight");
+        if (c.int_weights && std::fabs((double)d.w[j] * c.int_scale) >= lim)
+            return set_error(NH_ERR_ARG, "weight x int_scale exceeds the int32 ring");
+        if (d.tau && !(fin(d.tau[j]) && d.tau[j] >= 0.0f)) return set_error(NH_ERR_ARG, "synapse tau must be finite and >= 0");
+        maxd = std::max<uint32_t>(maxd, d.delay[j]);
+    }
+    if (d.n_inputs && d.stim_ptr) {
+        if (!d.stim_input && d.stim_ptr[d.n]) return set_error(NH_ERR_ARG, "stimulus arrays missing");
+        if (d.stim_ptr[0] != 0) return set_error(NH_ERR_ARG, "stim_ptr[0] must be 0");
+        for (uint32_t i = 0; i < d.n; ++i) {
+            if (d.stim_ptr[i + 1] < d.stim_ptr[i]) return set_error(NH_ERR_ARG, "stim_ptr not monotone");
+            for (uint32_t j = d.stim_ptr[i]; j < d.stim_ptr[i + 1]; ++j) {
+                if (d.stim_input[j] >= d.n_inputs) return set_error(NH_ERR_ARG, "stimulus input index out of range");
+                if (!d.stim_w || !fin(d.stim_w[j])) return set_error(NH_ERR_ARG, "non-finite stimulus weight");
+            }
+        }
+    }
+    if (max_delay) *max_delay = maxd;
+    return NH_OK;
+}
+
 int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, uint32_t raster_steps,
                  const Tuning &tune) {
     if (nn == 0) return set_error(NH_ERR_ARG, "no networks");
@@ -86,46 +155,15 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         const nh_net_desc &d = nets[k];
         if ((cfgs[k].int_weights ? 1 : 0) != ring_int)
             return set_error(NH_ERR_ARG, "all packed networks must share int_weights");
-        if (!(cfgs[k].dt_ms > 0.0f) || !std::isfinite(cfgs[k].dt_ms))
-            return set_error(NH_ERR_ARG, "dt_ms must be positive");
-        if (d.n && (!d.src_ptr || !d.model || !d.p || !d.x0))
-            return set_error(NH_ERR_ARG, "network arrays missing");
-        if (d.n_models == 0 && d.n) return set_error(NH_ERR_ARG, "empty model table");
-        if (d.n_models > 8) return set_error(NH_ERR_ARG, "model table too large");
-        if (d.n && d.n_state != 2) return set_error(NH_ERR_ARG, "state row width must be 2");
-        if (d.n_params > (uint32_t)kMaxParams) return set_error(NH_ERR_ARG, "param row too wide");
-        for (uint32_t m = 0; m < d.n_models; ++m) {
-            if (d.model_ids[m] > 1)
-                return set_error(NH_ERR_ARG, "unknown model id " + std::to_string(d.model_ids[m]));
-        }
+        uint32_t dk = 1;
+        TRY(check_desc(d, cfgs[k], &dk));
+        maxd = std::max(maxd, dk);
         for (uint32_t i = 0; i < d.n; ++i) {
-            if (d.model[i] >= d.n_models)
-                return set_error(NH_ERR_ARG, "neuron " + std::to_string(i) +
-                                                 " names model " + std::to_string(d.model[i]) +
-                                                 " outside the model table");
             const uint8_t id = d.model_ids[d.model[i]];
             if (id == 0) any_izh = true;
             else any_adex = true;
-            if (id == 1 && d.n_params < 10)
-                return set_error(NH_ERR_ARG, "adex rows need 10 params");
-            if (id == 0 && d.n_params < 4) return set_error(NH_ERR_ARG, "izh rows need 4 params");
         }
-        if (d.n && d.src_ptr[0] != 0) return set_error(NH_ERR_ARG, "src_ptr[0] must be 0");
-        for (uint32_t i = 0; i < d.n; ++i)
-            if (d.src_ptr[i + 1] < d.src_ptr[i]) return set_error(NH_ERR_ARG, "src_ptr not monotone");
-        const uint64_t E = d.n ? d.src_ptr[d.n] : 0;
-        if (E != d.n_syn) return set_error(NH_ERR_ARG, "src_ptr[n] != n_syn");
-        for (uint64_t j = 0; j < E; ++j) {
-            if (d.tgt[j] >= d.n) return set_error(NH_ERR_ARG, "synapse target out of range");
-            if (d.delay[j] < 1) return set_error(NH_ERR_ARG, "delay must be >= 1 step");
-            maxd = std::max<uint32_t>(maxd, d.delay[j]);
-        }
-        if (d.n_inputs && d.stim_ptr) {
-            for (uint32_t i = 0; i < d.n; ++i)
-                for (uint32_t j = d.stim_ptr[i]; j < d.stim_ptr[i + 1]; ++j)
-                    if (d.stim_input[j] >= d.n_inputs)
-                        return set_error(NH_ERR_ARG, "stimulus input index out of range");
-        }
+        const uint64_t E = d.n_syn;
         P = std::max(P, d.n_params);
         total_syn += E;
         slots[k].base = (uint32_t)off;

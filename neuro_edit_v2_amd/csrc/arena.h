@@ -13,6 +13,7 @@
 
 namespace nh {
 
+int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay);
 void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool push,
                  bool packed, hipStream_t st);
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st);

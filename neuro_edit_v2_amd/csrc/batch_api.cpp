@@ -88,6 +88,7 @@ extern "C" {
 int nh_batch_create(uint32_t n_nets, const nh_net_desc *nets, const nh_config *cfgs, nh_batch **out) {
     if (!nets || !cfgs || !out) return set_error(NH_ERR_ARG, "null argument");
     *out = nullptr;
+    for (uint32_t k = 0; k < n_nets; ++k) TRY(nh_check_net(&nets[k], &cfgs[k]));
     int dev = 0;
     if (hipGetDeviceCount(&dev) != hipSuccess || dev < 1) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
     std::unique_ptr<nh_batch> b(new (std::nothrow) nh_batch());

@@ -532,6 +532,7 @@ void nh_server_destroy(nh_server *s) {
 int nh_server_submit(nh_server *s, const nh_net_desc *d, const nh_config *cfg, const float *stim,
                      uint64_t stim_steps, uint64_t total_steps, uint32_t win_steps, uint64_t *id) {
     if (!s || !d || !cfg || !id) return set_error(NH_ERR_ARG, "null argument");
+    TRY(nh_check_net(d, cfg));
     if ((cfg->int_weights ? 1 : 0) != s->cfg.int_weights)
         return set_error(NH_ERR_ARG, "request int_weights differs from the server's ring");
     if (d->n_inputs > s->cfg.n_inputs) return set_error(NH_ERR_ARG, "too many inputs for the server");

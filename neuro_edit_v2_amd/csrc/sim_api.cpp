@@ -119,12 +119,18 @@ int nh_available(void) {
     return 1;
 }
 
+int nh_check_net(const nh_net_desc *net, const nh_config *cfg) {
+    if (!net || !cfg) return set_error(NH_ERR_ARG, "null argument");
+    if (!(cfg->dt_ms >= 0.001f && cfg->dt_ms <= 100.0f))
+        return set_error(NH_ERR_ARG, "dt_ms outside 0.001..100 (sim/src/lib.rs:35-39)");
+    return nh::check_desc(*net, *cfg, nullptr);
+}
+
 int nh_create(const nh_net_desc *net, const nh_config *cfg, nh_sim **out) {
     if (!net || !cfg || !out) return set_error(NH_ERR_ARG, "null argument");
     *out = nullptr;
+    TRY(nh_check_net(net, cfg));
     if (!nh_available()) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
-    if (!(cfg->dt_ms >= 0.001f && cfg->dt_ms <= 100.0f))
-        return set_error(NH_ERR_ARG, "dt_ms outside 0.001..100 (sim/src/lib.rs:35-39)");
     std::unique_ptr<nh_sim> s(new (std::nothrow) nh_sim());
     if (!s) return set_error(NH_ERR_CAPACITY, "out of host memory");
     s->cfg = *cfg;
