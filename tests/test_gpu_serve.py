@@ -78,3 +78,23 @@ def test_request_too_large_is_refused():
     with pytest.raises(NHError):
         srv.submit(net, cfg, ser, TOTAL, WIN)
     srv.close()
+
+
+def test_served_float_ring_close_to_alone():
+    """Float ring: the order of float atomic adds differs between the mix
+    and a request alone, so the bar is a tolerance -- per-request spike
+    totals within 2 % and per-neuron window counts within 2 % mean absolute
+    deviation (relative to the mean count)."""
+    srv = Server(cap_neurons=64 * 1024, cap_synapses=2_000_000, max_requests=8, n_windows=NWIN,
+                 chunk_steps=16, int_weights=0)
+    reqs = [request(s, e, ring=False) for s, e in [(21, 9), (22, 12), (23, 10), (24, 14)]]
+    ids = [srv.submit(net, cfg, ser, TOTAL, WIN) for net, cfg, ser in reqs]
+    got = [srv.wait(r) for r in ids]
+    assert srv.stats()["peak_active"] >= 2
+    for (net, cfg, ser), (counts, info) in zip(reqs, got):
+        want = alone(net, cfg, ser).astype(np.float64)
+        g = counts.astype(np.float64)
+        assert want.sum() > 0
+        assert abs(g.sum() - want.sum()) <= 0.02 * want.sum() + 1
+        assert np.abs(g - want).mean() <= 0.02 * max(want.mean(), 1e-9) + 1e-3
+    srv.close()
