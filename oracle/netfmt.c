@@ -162,6 +162,28 @@ static float pair_dist(const nb_neuron *a, const nb_neuron *c)
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
+/* builders::set_synapse_tau_dist (net-format/src/builders.rs:239-258):
+ * every synapse from a member of group sg to a member of group tg, in
+ * insertion order, gets tau = max(gauss(mean, sd), 0) as f32. */
+void or_netb_set_synapse_tau_dist(or_netb *b, uint32_t sg, uint32_t tg, double mean, double sd,
+                                  or_rng *r)
+{
+    if (sg >= b->ng || tg >= b->ng)
+        return;
+    uint8_t *in_s = (uint8_t *)calloc(b->n ? b->n : 1, 1), *in_t = (uint8_t *)calloc(b->n ? b->n : 1, 1);
+    for (uint32_t k = 0; k < b->gr[sg].len; ++k) in_s[b->gr[sg].m[k]] = 1;
+    for (uint32_t k = 0; k < b->gr[tg].len; ++k) in_t[b->gr[tg].m[k]] = 1;
+    for (uint64_t j = 0; j < b->ns; ++j) {
+        nb_syn *s = &b->sy[j];
+        if (in_s[s->src] && in_t[s->tgt]) {
+            double g = or_rng_gauss(r, mean, sd);
+            s->tau_ms = (float)(g > 0.0 ? g : 0.0);
+        }
+    }
+    free(in_s);
+    free(in_t);
+}
+
 void or_netb_push_synapse(or_netb *b, uint32_t src, uint32_t tgt, float w, float delay_ms,
                           float tau_ms)
 {

@@ -19,7 +19,8 @@
  *   net-format/src/rng.rs:17-81  xoshiro256** Rng
  *   net-format/src/network.rs:451-692  add_neuron, set_kind, connect,
  *                                conduction_delay_ms, synapse_csr
- *   net-format/src/builders.rs:11-296  grids, create_connections, exc/inh
+ *   net-format/src/builders.rs:11-296  grids, create_connections, exc/inh,
+ *                                set_synapse_tau_dist
  *   sim/src/bench.rs:10-53       bench_net
  *
  * Additions that the reference does not have (defined identically in the
@@ -153,6 +154,8 @@ uint32_t or_netb_grid3d(or_netb *b, uint32_t width, uint32_t height, uint32_t de
 void or_netb_create_connections(or_netb *b, uint32_t group, double scaling, double lambda,
                                 float weight, or_rng *r);
 void or_netb_set_exc_inh(or_netb *b, uint32_t group, double inh_fraction, or_rng *r);
+void or_netb_set_synapse_tau_dist(or_netb *b, uint32_t src_group, uint32_t tgt_group, double mean,
+                                  double sd, or_rng *r);
 void or_netb_set_exc_inh_exact(or_netb *b, uint32_t group, uint64_t n_inh, or_rng *r);
 void or_netb_bench_net(or_netb *b, uint32_t n, uint32_t k, uint64_t seed, float w_scale,
                        float w_inh);

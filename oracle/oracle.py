@@ -109,6 +109,7 @@ def lib():
         "or_netb_create_connections": (None, [vp, u32, f64, f64, f32, C.POINTER(RngState)]),
         "or_netb_set_exc_inh": (None, [vp, u32, f64, C.POINTER(RngState)]),
         "or_netb_set_exc_inh_exact": (None, [vp, u32, u64, C.POINTER(RngState)]),
+        "or_netb_set_synapse_tau_dist": (None, [vp, u32, u32, f64, f64, C.POINTER(RngState)]),
         "or_netb_bench_net": (None, [vp, u32, u32, u64, f32, f32]),
         "or_netb_add_input": (u32, [vp, u32, u32p, f32p]),
         "or_netb_n": (u32, [vp]),
@@ -219,6 +220,9 @@ class NetBuilder:
 
     def set_exc_inh(self, group, frac, rng):
         lib().or_netb_set_exc_inh(self.h, group, frac, C.byref(rng.st))
+
+    def set_synapse_tau_dist(self, src_group, tgt_group, mean, sd, rng):
+        lib().or_netb_set_synapse_tau_dist(self.h, src_group, tgt_group, mean, sd, C.byref(rng.st))
 
     def set_exc_inh_exact(self, group, n_inh, rng):
         lib().or_netb_set_exc_inh_exact(self.h, group, n_inh, C.byref(rng.st))

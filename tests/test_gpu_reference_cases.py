@@ -1,0 +1,104 @@
+"""The reference's own GPU acceptance tests (sim/tests/gpu_vs_cpu.rs),
+case for case, with the same networks, configs and bars; the CPU side is
+the oracle.  (The v-watch and mixed-AdEx cases of v_watch.rs and adex.rs
+are in test_gpu_api.py.)"""
+import numpy as np
+import pytest
+
+from neuro_edit_v2_amd import Sim, SimConfig
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def lsm_like_net(n_side, seed, tau_dist=None):
+    """gpu_vs_cpu.rs:11-18: 3D grid spacing 60, exact 20 % inhibitory,
+    create_connections(0.4, 100, 5.0); optionally
+    set_synapse_tau_dist(liquid, liquid, mean, sd) (builders.rs:239-258)."""
+    b = O.NetBuilder()
+    rng = O.Rng(seed)
+    g = b.grid3d(n_side, n_side, n_side, 60.0)
+    b.set_exc_inh_exact(g, n_side ** 3 // 5, rng)
+    b.create_connections(g, 0.4, 100.0, 5.0, rng)
+    if tau_dist is not None:
+        b.set_synapse_tau_dist(g, g, tau_dist[0], tau_dist[1], O.Rng(tau_dist[2]))
+    return b.build(0.1)
+
+
+def pair(net, **kw):
+    return Sim(net, SimConfig(record_spikes=True, **kw)), O.OracleSim(net, O.default_cfg(**kw))
+
+
+def test_gpu_matches_cpu_spike_counts_quiet():
+    """gpu_vs_cpu.rs:24-48: 8 driven neurons, no noise, 50 ms: equal totals."""
+    net = lsm_like_net(8, 42)
+    gpu, cpu = pair(net)
+    for i in range(8):
+        gpu.set_external(i, 12.0)
+        cpu.set_external(i, 12.0)
+    gpu.run_ms(50.0)
+    cpu.run_ms(50.0)
+    print(f"cpu spikes={cpu.spikes_total()} gpu spikes={gpu.spikes_total()}")
+    assert cpu.spikes_total() > 0
+    assert gpu.spikes_total() == cpu.spikes_total()
+
+
+def test_gpu_matches_cpu_with_custom_tau():
+    """gpu_vs_cpu.rs:52-78: recurrent synapses with tau ~ N(25, 10) ms."""
+    net = lsm_like_net(8, 42, tau_dist=(25.0, 10.0, 9))
+    n_custom = int((net["tau"] > 0).sum())
+    assert n_custom > 0
+    gpu, cpu = pair(net)
+    for i in range(8):
+        gpu.set_external(i, 12.0)
+        cpu.set_external(i, 12.0)
+    gpu.run_ms(50.0)
+    cpu.run_ms(50.0)
+    print(f"custom tau: cpu={cpu.spikes_total()} gpu={gpu.spikes_total()} (C={n_custom})")
+    assert cpu.spikes_total() > 0
+    assert gpu.spikes_total() == cpu.spikes_total()
+
+
+def test_gpu_cpu_close_with_noise():
+    """gpu_vs_cpu.rs:81-111: 1 kHz x 20 mV kicks, 100 ms: totals within 25 %,
+    trace in [0, 1], v in (-200, 100)."""
+    net = lsm_like_net(8, 7)
+    gpu, cpu = pair(net, noise_rate_hz=1000.0, noise_kick_mv=20.0, seed=99)
+    gpu.run_ms(100.0)
+    cpu.run_ms(100.0)
+    a, g = cpu.spikes_total(), gpu.spikes_total()
+    print(f"cpu={a} gpu={g}")
+    assert a > 0
+    assert abs(a - g) / max(a, 1) < 0.25
+    tr = gpu.trace()
+    assert np.all((tr >= 0.0) & (tr <= 1.0))
+    v = gpu.v()
+    assert np.all((v < 100.0) & (v > -200.0))
+
+
+def test_gpu_int_weights_bitwise_deterministic():
+    """gpu_vs_cpu.rs:115-141: two identical int-ring runs, bitwise equal v
+    and 64-step spike counts."""
+    net = lsm_like_net(8, 5)
+    kw = dict(noise_rate_hz=2000.0, noise_kick_mv=20.0, seed=7, int_weights=True)
+    a = Sim(net, SimConfig(record_spikes=True, **kw))
+    b = Sim(net, SimConfig(record_spikes=True, **kw))
+    a.run_ms(80.0)
+    b.run_ms(80.0)
+    assert a.spikes_total() == b.spikes_total() > 0
+    assert np.array_equal(a.v(), b.v())
+    assert np.array_equal(a.spike_counts(64), b.spike_counts(64))
+
+
+def test_gpu_int_weights_close_to_float():
+    """gpu_vs_cpu.rs:145-177: int ring within 30 % of the float ring."""
+    net = lsm_like_net(8, 11)
+    base = dict(noise_rate_hz=2000.0, noise_kick_mv=20.0, seed=3)
+    f = Sim(net, SimConfig(**base))
+    i = Sim(net, SimConfig(int_weights=True, **base))
+    f.run_ms(80.0)
+    i.run_ms(80.0)
+    tf, ti = f.spikes_total(), i.spikes_total()
+    print(f"float={tf} int={ti}")
+    assert tf > 0 and ti > 0
+    assert abs(tf - ti) / max(tf, 1) < 0.3

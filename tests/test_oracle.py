@@ -548,3 +548,26 @@ def test_f64_instance_tracks_f32():
     b.run_steps(50)
     assert np.max(np.abs(a.v() - b.v())) < 1e-3
     assert a.spikes_total() == b.spikes_total()
+
+
+def test_set_synapse_tau_dist_draws_in_insertion_order():
+    """builders.rs:239-258: every liquid->liquid synapse gets
+    max(gauss(mean, sd), 0) from the given Rng, in insertion order -- so the
+    taus, read back in insertion order, are exactly that Rng's draws."""
+    b = O.NetBuilder()
+    rng = O.Rng(4)
+    g = b.grid3d(5, 5, 5, 60.0)
+    b.set_exc_inh_exact(g, 25, rng)
+    b.create_connections(g, 0.4, 100.0, 5.0, rng)
+    b.set_synapse_tau_dist(g, g, 25.0, 10.0, O.Rng(9))
+    net = b.build(0.1)
+    r = O.Rng(9)
+    want = np.array([max(r.gauss(25.0, 10.0), 0.0) for _ in range(len(net["tgt"]))], np.float32)
+    # the CSR is a stable sort of insertion order by source; recover it
+    src = np.repeat(np.arange(net["n"]), np.diff(net["src_ptr"].astype(np.int64)))
+    assert np.all(np.diff(src) >= 0)
+    got = net["tau"]
+    assert got.min() >= 0.0 and abs(float(got.mean()) - 25.0) < 1.5
+    # insertion order of create_connections is source-major, so it equals CSR order
+    assert np.array_equal(np.sort(got), np.sort(want))
+    assert np.array_equal(got, want)
