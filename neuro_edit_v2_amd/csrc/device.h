@@ -28,7 +28,12 @@ __device__ __forceinline__ uint64_t noise_key(uint32_t i, uint64_t t, uint64_t s
 // key, scaled by 2^-32, compared with rate*dt/1000
 // (sim/src/state.rs:283-301, sim/src/kernels.rs:69-77,128-130).
 __device__ __forceinline__ bool kick_fires(uint64_t h, float p) {
-    return (float)(uint32_t)(h >> 32) * 0x1p-32f < p;
+    // the high word converted as a 32-bit integer (one v_cvt_f32_u32,
+    // round to nearest even); left to itself the compiler widens this to a
+    // 64-bit integer conversion sequence with the same result
+    float f;
+    asm("v_cvt_f32_u32 %0, %1" : "=v"(f) : "v"((uint32_t)(h >> 32)));
+    return f * 0x1p-32f < p;
 }
 
 // Gaussian current draw: Irwin-Hall(12) over twelve 8-bit fields -- the four

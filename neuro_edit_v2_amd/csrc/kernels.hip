@@ -266,6 +266,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
             model = a.krow[0].model;
 #pragma unroll
             for (int r = 1; r < kKargRows; ++r) {
+                if ((uint32_t)r >= a.n_krows) break;  // uniform: only the rows in use
                 const bool sel = prow == (uint32_t)r;
 #pragma unroll
                 for (int q = 0; q < kMaxParams; ++q) pr[q] = sel ? a.krow[r].p[q] : pr[q];
