@@ -64,34 +64,36 @@ def bench_batched(args):
     from neuro_edit_v2_amd import workloads as W
     from neuro_edit_v2_amd.batch import Batch
 
-    steps, dt = 1000, 0.5
-    nets, series = W.batch_nets(args.batch_nets, dt=dt, steps=steps)
+    steps, warm, dt = 1000, 100, 0.5
+    nets, series = W.batch_nets(args.batch_nets, dt=dt, steps=warm + steps)
     cfgs = [SimConfig(dt_ms=dt, synapse_tau_ms=5.0, seed=k + 1, int_weights=True,
-                      int_scale=65536.0, raster_steps=steps) for k in range(len(nets))]
+                      int_scale=65536.0, raster_steps=warm + steps) for k in range(len(nets))]
     b = Batch(nets, cfgs)
     for k in range(len(nets)):
         for q in range(8):
             b.play_input(k, q, series[k][:, q])
-    b.run_steps(0)                       # capture the step graph before timing
+    b.run_steps(warm)                    # untimed warm-up (graph capture, first touches)
     b.synchronize()
+    bev0 = b.events_total()
     t0 = time.perf_counter()
     b.run_steps(steps)
     b.synchronize()
     bwall = time.perf_counter() - t0
     bgpu = b.last_run_ms()
-    bev = b.events_total()
+    bev = b.events_total() - bev0
     plain_wall, plain_ev, same, spikes = 0.0, 0, True, 0
     for k, net in enumerate(nets):
         s = Sim(net, cfgs[k])
         for q in range(8):
             s.play_input(q, series[k][:, q])
-        s.run_steps(0)
+        s.run_steps(warm)
         s.synchronize()
+        pev0 = s.events_total()
         t1 = time.perf_counter()
         s.run_steps(steps)
         s.synchronize()
         plain_wall += time.perf_counter() - t1
-        plain_ev += s.events_total()
+        plain_ev += s.events_total() - pev0
         _, pc, pi = s.read_spikes()
         _, bc, bi = b.read_spikes(k)
         same = same and np.array_equal(pc, bc) and np.array_equal(pi, bi)
@@ -100,7 +102,9 @@ def bench_batched(args):
     n_tot = sum(int(n["n"]) for n in nets)
     return {"workload": f"{len(nets)} grid liquids, {n_tot} neurons, "
                         f"{sum(len(n['tgt']) for n in nets)} synapses, half izh / half adex, "
-                        f"8 Bernoulli(0.05) 1 ms inputs each, {steps} steps (500 ms), int ring",
+                        f"8 Bernoulli(0.05) 1 ms inputs each, {steps} steps (500 ms) timed after "
+                        f"{warm} warm-up steps, int ring",
+            "batched_us_per_step": round(bgpu / steps * 1e3, 3),
             "batched_events_per_s": round(bev / bwall, 1),
             "plain_sequential_events_per_s": round(plain_ev / plain_wall, 1),
             "speedup": round((bev / bwall) / (plain_ev / plain_wall), 2),

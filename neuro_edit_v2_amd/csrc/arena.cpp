@@ -499,6 +499,11 @@ StepArgs Arena::args() const {
     a.x = d_x.as<float2>();
     a.s = d_s.as<float>();
     a.ring = d_ring.p;
+    {
+        const uint64_t rb = (uint64_t)D * n_total * 4;
+        a.ring_buf = rb < 0x80000000ull ? 1u : 0u;   // the dropped-slot offset must be out of range
+        a.ring_bytes = a.ring_buf ? (uint32_t)rb : 0u;
+    }
     a.ext = has_ext ? d_ext.as<float>() : nullptr;
     a.n_krows = pmode == 0 ? (uint32_t)krows_.size() : 0;
     for (size_t r = 0; r < krows_.size() && r < (size_t)kKargRows; ++r) a.krow[r] = krows_[r];

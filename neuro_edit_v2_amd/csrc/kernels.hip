@@ -9,7 +9,8 @@
 //  * k_step<PUSH=true> ("fused" shape): each wave updates its 64 neurons,
 //    ballots its spikes (the wave64 mask is one word of the spike bitmap),
 //    then pushes the out-synapse rows of its own spiking neurons into the
-//    ring with hardware atomics (lanes stride the delay-sorted row).  No
+//    ring with hardware atomics: the rows are walked as one concatenated
+//    list, every load of a 256-entry batch issued before its atomics.  No
 //    cross-wave dependency exists inside a step, so there is no grid
 //    barrier; the kernel boundary orders step t's pushes before step t+1's
 //    reads.
@@ -25,6 +26,28 @@
 #include "layout.h"
 
 namespace nh {
+
+// Raw buffer atomics (LLVM intrinsics): an access whose offset is outside
+// the descriptor's range is dropped by the hardware, which lets a batch of
+// deliveries skip its empty slots without a branch per slot.
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+__device__ int32_t buf_atomic_add_i32(int32_t v, i32x4 rsrc, int32_t voff, int32_t soff,
+                                      int32_t aux) __asm("llvm.amdgcn.raw.buffer.atomic.add.i32");
+__device__ float buf_atomic_fadd_f32(float v, i32x4 rsrc, int32_t voff, int32_t soff,
+                                     int32_t aux) __asm("llvm.amdgcn.raw.buffer.atomic.fadd.f32");
+constexpr uint32_t kBufDrop = 0x80000000u;  // out-of-range offset: the access is dropped
+
+// Raw (unstrided) buffer descriptor over `bytes` bytes at p (gfx9 family:
+// 32-bit data format in the third dword).
+__device__ __forceinline__ i32x4 buffer_rsrc(const void *p, uint32_t bytes) {
+    const uint64_t a = (uint64_t)p;
+    i32x4 r;
+    r.x = (int32_t)(uint32_t)a;
+    r.y = (int32_t)(uint32_t)(a >> 32);
+    r.z = (int32_t)bytes;
+    r.w = 0x00020000;
+    return r;
+}
 
 template <int RING> struct Ring;
 // Ring read scale: x / scale (sim/src/kernels.rs:60), or x * 2^-k when the
@@ -43,6 +66,9 @@ template <> struct Ring<0> {
     using T = float;
     static __device__ __forceinline__ float get(T v, const RingScale &) { return v; }
     static __device__ __forceinline__ void add(T *p, T v) { unsafeAtomicAdd(p, v); }
+    static __device__ __forceinline__ void add_buf(i32x4 r, uint32_t off, T v) {
+        (void)buf_atomic_fadd_f32(v, r, (int32_t)off, 0, 0);
+    }
 };
 template <> struct Ring<1> {
     using T = int32_t;
@@ -54,6 +80,9 @@ template <> struct Ring<1> {
         return (float)v / rs.scale;
     }
     static __device__ __forceinline__ void add(T *p, T v) { atomicAdd(p, v); }
+    static __device__ __forceinline__ void add_buf(i32x4 r, uint32_t off, T v) {
+        (void)buf_atomic_add_i32(v, r, (int32_t)off, 0, 0);
+    }
 };
 
 __device__ __forceinline__ uint32_t wave_rank(uint64_t mask) {
@@ -70,12 +99,11 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
 constexpr uint32_t kPushUnroll = 4;
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;  // batch slot without an entry
 
-// One ring delivery: synapse j of the delay-sorted rows (target and delay
-// packed in one word, or target plus a separate 16-bit delay).
-template <int RING, bool PACKED>
-__device__ __forceinline__ void deliver(const StepArgs &a, uint32_t sy, uint32_t d16,
-                                        typename Ring<RING>::T wv, uint32_t slot0) {
-    using RT = typename Ring<RING>::T;
+// Ring slot and target of one delivery: synapse word sy (target and delay
+// packed in one word, or target plus a separate 16-bit delay d16).
+template <bool PACKED>
+__device__ __forceinline__ uint64_t ring_index(const StepArgs &a, uint32_t sy, uint32_t d16,
+                                               uint32_t slot0) {
     uint32_t tgt, d;
     if (PACKED) {
         tgt = sy & 0x07FFFFFFu;
@@ -86,11 +114,15 @@ __device__ __forceinline__ void deliver(const StepArgs &a, uint32_t sy, uint32_t
     }
     uint32_t sl = slot0 + d;
     sl = sl >= a.D ? sl - a.D : sl;
-    Ring<RING>::add((RT *)a.ring + (size_t)sl * a.n_total + tgt, wv);
+    return (uint64_t)sl * a.n_total + tgt;
 }
 
 // Load the entries j[0..kPushUnroll) (kNoEntry = none) and deliver them:
-// all loads first, then the atomics.
+// all loads first (unconditional: an empty slot reads entry 0), then the
+// atomics.  With a ring below 2 GB the atomics are buffer atomics whose
+// empty slots carry an out-of-range offset, so the batch has no branch
+// per slot (branches there make the compiler wait for each atomic before
+// the next); otherwise each slot's atomic is guarded.
 template <int RING, bool PACKED>
 __device__ __forceinline__ void deliver_batch(const StepArgs &a, const uint32_t (&j)[kPushUnroll],
                                               uint32_t slot0) {
@@ -100,15 +132,25 @@ __device__ __forceinline__ void deliver_batch(const StepArgs &a, const uint32_t 
     RT wv[kPushUnroll];
 #pragma unroll
     for (uint32_t r = 0; r < kPushUnroll; ++r) {
-        if (j[r] != kNoEntry) {
-            sy[r] = a.syn[j[r]];
-            wv[r] = w[j[r]];
-            d16[r] = PACKED ? 0u : a.delay16[j[r]];
-        }
+        const uint32_t jr = j[r] != kNoEntry ? j[r] : 0u;
+        sy[r] = a.syn[jr];
+        wv[r] = w[jr];
+        d16[r] = PACKED ? 0u : a.delay16[jr];
     }
+    if (a.ring_buf) {
+        const i32x4 rs = buffer_rsrc(a.ring, a.ring_bytes);
 #pragma unroll
-    for (uint32_t r = 0; r < kPushUnroll; ++r)
-        if (j[r] != kNoEntry) deliver<RING, PACKED>(a, sy[r], d16[r], wv[r], slot0);
+        for (uint32_t r = 0; r < kPushUnroll; ++r) {
+            // ring below 2 GB: the byte offset fits in 31 bits
+            const uint32_t off = (uint32_t)ring_index<PACKED>(a, sy[r], d16[r], slot0) * 4u;
+            Ring<RING>::add_buf(rs, j[r] != kNoEntry ? off : kBufDrop, wv[r]);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t r = 0; r < kPushUnroll; ++r)
+            if (j[r] != kNoEntry)
+                Ring<RING>::add((RT *)a.ring + ring_index<PACKED>(a, sy[r], d16[r], slot0), wv[r]);
+    }
 }
 
 // Push one spike's out-synapse row into the ring; lanes stride the row
@@ -228,13 +270,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
     const uint64_t t = *a.t_dev + a.k;
+    // raster window origin, read here (before any store of this kernel) so
+    // it is a scalar load issued with the others, not a dependent vector
+    // load after the state stores
+    const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
 
     const uint32_t local = i - nr.base;
     const uint64_t tl = t - (uint64_t)nr.t0;
     const bool act = local < nr.n && nr.active && (nr.t_len == 0 || tl < nr.t_len);
     bool spiked = false;
+    float s_new = s_old;
     if (act) {
-        ring[ri] = (RT)0;
         const RingScale rsc(nr.int_scale);
         const float arr = Ring<RING>::get(rv, rsc);
         const float sv = s_old + arr;
@@ -301,26 +347,48 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
             spiked = adex_step(pr, xs, I, nr.dt);
         else
             spiked = izh_step(pr[0], pr[1], pr[2], pr[3], xs, I, nr.dt);
-        a.x[i] = xs;
-        a.s[i] = sv * nr.decay_s;
-        if (spiked) {
-            a.pn_total[i] += 1;
-            a.last_spike[i] = t < 0xFFFFFFFEull ? (uint32_t)t : 0xFFFFFFFEu;
-            if (a.wcnt && nr.win_steps) {
-                const uint64_t wi = tl / nr.win_steps;
-                if (wi < a.n_win) a.wcnt[(size_t)wi * a.n_total + i] += 1;
-            }
-        }
+        s_new = sv * nr.decay_s;
     }
 
     // ---- wave64 ballot: the mask IS this wave's word of the spike bitmap
     const uint64_t mask = __ballot(spiked);
+    if (PUSH && mask) {
+        // The push goes first, before any store of this step: a wave's
+        // vector memory operations complete in order, so rows read after
+        // the stores would wait for the stores.  The push writes slots
+        // t+1..t+D-1 only, never the slot cleared below.
+        push_wave<RING, PACKED>(a, mask, row_lo, row_hi, slot, lane);
+        if (a.xs_lo) {
+            // custom-tau rows (rare): one spike at a time
+            for (uint64_t m = mask; m;) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                push_custom<RING>(a, wave * kWave + b, slot, lane);
+            }
+        }
+    }
+    // ---- this step's stores
+    if (act) {
+        ring[ri] = (RT)0;
+        a.x[i] = xs;
+        a.s[i] = s_new;
+        if (spiked) {
+            // no-return atomics: nothing waits on a read of the old count
+            // (each neuron owns its counter, so there is no contention)
+            atomicAdd(&a.pn_total[i], 1u);
+            a.last_spike[i] = t < 0xFFFFFFFEull ? (uint32_t)t : 0xFFFFFFFEu;
+            if (a.wcnt && nr.win_steps) {
+                const uint64_t wi = tl / nr.win_steps;
+                if (wi < a.n_win) atomicAdd(&a.wcnt[(size_t)wi * a.n_total + i], 1u);
+            }
+        }
+    }
     if (lane == 0) {
         if (a.bits) a.bits[(size_t)(t % a.bits_rows) * a.words + wave] = mask;
         if (a.rast_bits) {
             // raster window row of this step; the host drains the window
             // before it fills, the flag reports a step outside it
-            const uint64_t rs = t - a.rast_ctl[1];
+            const uint64_t rs = t - rast_origin;
             if (rs < a.rast_cap_steps) a.rast_bits[(size_t)rs * a.words + wave] = mask;
             else a.rast_ctl[2] = 1;
         }
@@ -339,17 +407,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
         if (lane == 0) off = atomicAdd(&a.cnt[L], nsp);
         off = uniform(off);
         if (spiked) a.spk[(size_t)L * a.n_total + off + wave_rank(mask)] = i;
-    }
-    if (PUSH) {
-        push_wave<RING, PACKED>(a, mask, row_lo, row_hi, slot, lane);
-        if (a.xs_lo) {
-            // custom-tau rows (rare): one spike at a time
-            for (uint64_t m = mask; m;) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(m);
-                m &= m - 1;
-                push_custom<RING>(a, wave * kWave + b, slot, lane);
-            }
-        }
     }
 }
 

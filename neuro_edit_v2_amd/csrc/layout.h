@@ -63,6 +63,8 @@ struct StepArgs {
     float2 *x;                   // (v, u|w) per neuron
     float *s;                    // decaying synaptic current
     void *ring;                  // D x n_total (float or int32)
+    uint32_t ring_bytes;         // ring size when it is below 2 GB (buffer atomics)
+    uint32_t ring_buf;           // 1: deliveries use buffer atomics over ring_bytes
     const float *ext;            // external current or null
     // parameters
     uint32_t n_krows;            // rows in krow (1..kKargRows) in kernel-argument mode
