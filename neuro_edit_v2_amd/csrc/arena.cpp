@@ -409,14 +409,9 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         TRY(upload(d_stim_input, stim_input));
         TRY(upload(d_stim_w, stim_w));
     }
-    stim_cap = 1024;
-    if (stim_cols) {
-        TRY(alloc(d_stim_win, (size_t)stim_cap * stim_cols * 4));
-        for (int i = 0; i < 2; ++i) {
-            HIPCHK(hipHostMalloc((void **)&h_stage[i], (size_t)stim_cap * stim_cols * 4, 0));
-            HIPCHK(hipEventCreateWithFlags(&stage_ev[i], hipEventDisableTiming));
-        }
-    }
+    stim_group.resize(nn);
+    for (uint32_t k = 0; k < nn; ++k) stim_group[k] = make_uint2(slots[k].stim_base, slots[k].n_inputs);
+    TRY(alloc_stim(1024));
     TRY(alloc(d_spk, (size_t)3 * std::max<uint32_t>(n_total, 1) * 4));
     TRY(alloc(d_cnt, 4 * 4));
     bits_rows = 0;
@@ -518,6 +513,8 @@ StepArgs Arena::args() const {
     a.stim_input = d_stim_input.as<uint32_t>();
     a.stim_w = d_stim_w.as<float>();
     a.stim_win = d_stim_win.as<float>();
+    a.stim_nz = d_stim_nz.as<uint32_t>();
+    a.stim_groups = (uint32_t)stim_group.size();
     a.stim_cols = stim_cols;
     a.spk = d_spk.as<uint32_t>();
     a.cnt = d_cnt.as<uint32_t>();
@@ -615,6 +612,7 @@ int Arena::check_args(const StepArgs &a) const {
         TRY(need(a.stim_input, "stim_input"));
         TRY(need(a.stim_w, "stim_w"));
         TRY(need(a.stim_win, "stim_win"));
+        TRY(need(a.stim_nz, "stim_nz"));
     }
     if (a.n_x) {
         TRY(need(a.xt_ptr, "xt_ptr"));
@@ -742,13 +740,40 @@ int Arena::capture_graph() {
     return NH_OK;
 }
 
+int Arena::alloc_stim(uint32_t cap) {
+    stim_cap = cap;
+    if (!stim_cols) return NH_OK;
+    const size_t ng = std::max<size_t>(stim_group.size(), 1);
+    TRY(alloc(d_stim_win, (size_t)stim_cap * stim_cols * 4));
+    TRY(alloc(d_stim_nz, (size_t)stim_cap * ng * 4));
+    for (int i = 0; i < 2; ++i) {
+        // window rows, then the group flags of every row
+        HIPCHK(hipHostMalloc((void **)&h_stage[i], ((size_t)stim_cap * stim_cols + stim_cap * ng) * 4, 0));
+        HIPCHK(hipEventCreateWithFlags(&stage_ev[i], hipEventDisableTiming));
+    }
+    return NH_OK;
+}
+
 int Arena::upload_stim(uint64_t t_arena, uint32_t k) {
     if (!stim_cols || !packer_) return NH_OK;
     float *h = h_stage[stage_i];
     if (stage_used[stage_i]) HIPCHK(hipEventSynchronize(stage_ev[stage_i]));
     stage_used[stage_i] = true;
     packer_(packer_ctx_, t_arena, k, h);
+    // per step and group: does any input differ from +-0?  (a step whose
+    // inputs are all zero adds only +-0 to I, which the kernel may skip)
+    const size_t ng = stim_group.size();
+    uint32_t *nz = (uint32_t *)(h + (size_t)stim_cap * stim_cols);
+    for (uint32_t j = 0; j < k; ++j) {
+        const float *row = h + (size_t)j * stim_cols;
+        for (size_t g = 0; g < ng; ++g) {
+            uint32_t any = 0;
+            for (uint32_t q = 0; q < stim_group[g].y; ++q) any |= row[stim_group[g].x + q] != 0.0f ? 1u : 0u;
+            nz[(size_t)j * ng + g] = any;
+        }
+    }
     HIPCHK(hipMemcpyAsync(d_stim_win.p, h, (size_t)k * stim_cols * 4, hipMemcpyHostToDevice, stream));
+    if (ng) HIPCHK(hipMemcpyAsync(d_stim_nz.p, nz, (size_t)k * ng * 4, hipMemcpyHostToDevice, stream));
     HIPCHK(hipEventRecord(stage_ev[stage_i], stream));
     stage_i ^= 1;
     return NH_OK;

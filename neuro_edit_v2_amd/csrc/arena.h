@@ -82,6 +82,12 @@ class Arena {
     uint32_t neuron_blocks() const;
     uint32_t scat_blocks() const;
     int upload_stim(uint64_t t_arena, uint32_t k);
+    // Stimulus window buffers for stim_cap steps of stim_cols columns, plus
+    // the per-step "inputs not all zero" flag of each column group (a
+    // network's inputs: group g = columns [stim_group[g].x, +stim_group[g].y)).
+    int alloc_stim(uint32_t cap);
+    std::vector<uint2> stim_group;
+    DBuf d_stim_nz;
 
     // Stimulus: the caller supplies a packer producing [k][stim_cols].
     using StimPacker = void (*)(void *ctx, uint64_t t_arena, uint32_t k, float *out);

@@ -249,10 +249,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     // the kernel arguments or the table makes every field a separate vector
     // load at its point of use).
     NetRec nr;
+    uint32_t net = 0;
     if (a.single_net) {
         nr = a.net0;
     } else {
-        nr = a.nets[uniform((uint32_t)a.wave_net[wave])];
+        net = uniform((uint32_t)a.wave_net[wave]);
+        nr = a.nets[net];
     }
 
     // ---- first batch of independent loads
@@ -274,6 +276,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     // it is a scalar load issued with the others, not a dependent vector
     // load after the state stores
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
+    // does this network have a nonzero input this step?  (scalar load,
+    // issued with the others)
+    const bool stim_on = a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
 
     const uint32_t local = i - nr.base;
     const uint64_t tl = t - (uint64_t)nr.t0;
@@ -299,7 +304,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
                 a.xsyn[kx] = xv * a.xdecay[kx];
             }
         }
-        if (a.stim_lo) {
+        if (stim_on) {
+            // Stimulus (stim.rs:119-127).  Skipped when every input of the
+            // network is zero this step: each term is then w * 0 = +-0, and
+            // I + (+-0) == I because I is never -0 here (ring sums start
+            // from +0 and never reach -0, so s + arrival is not -0, nor is
+            // that plus ext or plus a custom-tau trace).
             const float *row = a.stim_win + (size_t)a.k * a.stim_cols + nr.stim_base;
             for (uint32_t j = a.stim_lo[i]; j < a.stim_hi[i]; ++j)
                 I += a.stim_w[j] * row[a.stim_input[j]];

@@ -80,6 +80,8 @@ struct StepArgs {
     const uint32_t *stim_input;
     const float *stim_w;
     const float *stim_win;       // [chunk step][columns]
+    const uint32_t *stim_nz;     // [chunk step][group]: 0 = every input of the group is zero
+    uint32_t stim_groups;
     uint32_t stim_cols;
     // spike outputs
     uint32_t *spk;               // per-step spike lists (3 x n_total)

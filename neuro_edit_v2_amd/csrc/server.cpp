@@ -217,12 +217,10 @@ int build_arena(nh_server *s) {
         TRY(A(ar.d_stim_hi, n * 4));
         TRY(A(ar.d_stim_input, c.cap_stim * 4));
         TRY(A(ar.d_stim_w, c.cap_stim * 4));
-        ar.stim_cap = 1024;
-        TRY(A(ar.d_stim_win, (uint64_t)ar.stim_cap * ar.stim_cols * 4));
-        for (int i = 0; i < 2; ++i) {
-            HIPCHK(hipHostMalloc((void **)&ar.h_stage[i], (size_t)ar.stim_cap * ar.stim_cols * 4, 0));
-            HIPCHK(hipEventCreateWithFlags(&ar.stage_ev[i], hipEventDisableTiming));
-        }
+        // column group of slot q: its request's inputs
+        ar.stim_group.resize(c.max_requests);
+        for (uint32_t q = 0; q < c.max_requests; ++q) ar.stim_group[q] = make_uint2(q * c.n_inputs, c.n_inputs);
+        TRY(ar.alloc_stim(1024));
     } else {
         ar.stim_cap = 1024;
     }
