@@ -148,3 +148,34 @@ def test_batched_workload_full_size():
             assert np.array_equal(rc, bc) and np.array_equal(ri, bi), f"network {k} vs oracle"
     print(f"spikes {total}")
     assert total > 1000000
+
+
+def test_ring_above_2gb_uses_global_atomics_bit_exact():
+    """A delay ring of 32 slots x 17M neurons (2.2 GB) is beyond the buffer
+    descriptors the deliveries use below 2 GB, so the kernel takes its
+    global-atomic path: 40 steps, int ring, bit-exact against the oracle."""
+    n, steps = 17_000_000, 40
+    r = np.random.default_rng(5)
+    deg = 1
+    net = dict(n=n, src_ptr=np.arange(0, (n + 1) * deg, deg, dtype=np.uint32),
+               tgt=r.integers(0, n, n * deg, dtype=np.uint32),
+               w=r.uniform(0.5, 8.0, n * deg).astype(np.float32),
+               delay=r.integers(1, 32, n * deg).astype(np.uint16),
+               model_ids=np.zeros(1, np.uint8), model=np.zeros(n, np.uint8), n_params=4,
+               n_state=2, p=np.tile(np.float32([0.02, 0.2, -65.0, 8.0]), n),
+               x0=np.tile(np.float32([-65.0, -13.0]), n))
+    net["delay"][0] = 31                       # ring depth 32
+    kw = dict(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=400.0, noise_kick_mv=20.0, seed=9,
+              int_weights=True, int_scale=65536.0)
+    print("case: 17M neurons, 2.2 GB ring", flush=True)
+    sim = Sim(net, SimConfig(raster_steps=steps, **kw))
+    sim.run_steps(steps)
+    _, cnt, ids = sim.read_spikes()
+    v = sim.v()
+    sim.close()
+    ref = O.OracleSim(net, O.default_cfg(**kw))
+    rc, ri = ref.run_steps(steps, record=True)
+    print(f"spikes gpu={cnt.sum()} oracle={rc.sum()}")
+    assert rc.sum() > 10000
+    assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
+    assert np.array_equal(v, ref.v().astype(np.float32))
