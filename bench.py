@@ -54,11 +54,14 @@ def load_traffic(name):
     return None
 
 
-def bench_batched(args):
+def bench_batched(args, ring_int=True):
     """256 uneven networks (1k-20k neurons, half Izhikevich / half AdEx,
     own seeds, 8 Bernoulli(0.05) 1 ms bit-stream inputs each), 500 ms =
-    1000 steps, int ring: one packed arena vs the same networks launched one
-    at a time through the plain path; rasters compared per network."""
+    1000 steps: one packed arena (launch shape, workgroup size and graph
+    length picked by the autotuner) vs the same networks launched one at a
+    time through the plain path with default settings.  Int ring: rasters
+    compared per network (identical).  Float ring: the order of float ring
+    sums varies between runs, so per-network spike counts are compared."""
     import numpy as np
     from neuro_edit_v2_amd import Sim, SimConfig
     from neuro_edit_v2_amd import workloads as W
@@ -66,12 +69,18 @@ def bench_batched(args):
 
     steps, warm, dt = 1000, 100, 0.5
     nets, series = W.batch_nets(args.batch_nets, dt=dt, steps=warm + steps)
-    cfgs = [SimConfig(dt_ms=dt, synapse_tau_ms=5.0, seed=k + 1, int_weights=True,
+    cfgs = [SimConfig(dt_ms=dt, synapse_tau_ms=5.0, seed=k + 1, int_weights=ring_int,
                       int_scale=65536.0, raster_steps=warm + steps) for k in range(len(nets))]
     b = Batch(nets, cfgs)
     for k in range(len(nets)):
         for q in range(8):
             b.play_input(k, q, series[k][:, q])
+    tuned = None
+    if not args.no_autotune:
+        t_tune = time.perf_counter()
+        rep = b.autotune(128)            # trial runs from step 0, then resets
+        tuned = {"chosen": rep.strip().splitlines()[-1].replace("chosen: ", ""),
+                 "tune_s": round(time.perf_counter() - t_tune, 2)}
     b.run_steps(warm)                    # untimed warm-up (graph capture, first touches)
     b.synchronize()
     bev0 = b.events_total()
@@ -82,6 +91,7 @@ def bench_batched(args):
     bgpu = b.last_run_ms()
     bev = b.events_total() - bev0
     plain_wall, plain_ev, same, spikes = 0.0, 0, True, 0
+    dev = []
     for k, net in enumerate(nets):
         s = Sim(net, cfgs[k])
         for q in range(8):
@@ -97,13 +107,15 @@ def bench_batched(args):
         _, pc, pi = s.read_spikes()
         _, bc, bi = b.read_spikes(k)
         same = same and np.array_equal(pc, bc) and np.array_equal(pi, bi)
+        dev.append(abs(int(bc.sum()) - int(pc.sum())) / max(int(pc.sum()), 1))
         spikes += int(pc.sum())
         s.close()
     n_tot = sum(int(n["n"]) for n in nets)
-    return {"workload": f"{len(nets)} grid liquids, {n_tot} neurons, "
+    out = {"workload": f"{len(nets)} grid liquids, {n_tot} neurons, "
                         f"{sum(len(n['tgt']) for n in nets)} synapses, half izh / half adex, "
                         f"8 Bernoulli(0.05) 1 ms inputs each, {steps} steps (500 ms) timed after "
-                        f"{warm} warm-up steps, int ring",
+                        f"{warm} warm-up steps, {'int' if ring_int else 'float'} ring",
+            "autotune": tuned,
             "batched_us_per_step": round(bgpu / steps * 1e3, 3),
             "batched_events_per_s": round(bev / bwall, 1),
             "plain_sequential_events_per_s": round(plain_ev / plain_wall, 1),
@@ -111,8 +123,14 @@ def bench_batched(args):
             "batched_wall_s": round(bwall, 5), "batched_gpu_ms": round(bgpu, 3),
             "plain_wall_s": round(plain_wall, 5),
             "batched_sim_ms_per_s": round(len(nets) * steps * dt / bwall, 1),
-            "events": int(bev), "events_plain": int(plain_ev), "spikes": spikes,
-            "rasters_identical": bool(same and bev == plain_ev)}
+            "events": int(bev), "events_plain": int(plain_ev), "spikes": spikes}
+    if ring_int:
+        out["rasters_identical"] = bool(same and bev == plain_ev)
+    else:
+        # float sums in atomic order: per-network spike totals, batched vs plain
+        out["spike_count_rel_dev_mean"] = round(float(np.mean(dev)), 5)
+        out["spike_count_rel_dev_max"] = round(float(np.max(dev)), 5)
+    return out
 
 
 def bench_served(args, ring_int=True):
@@ -227,7 +245,8 @@ def main():
     ap.add_argument("--shape", type=int, default=-1, help="-1 auto, 0 two kernels, 1 fused")
     ap.add_argument("--exec", dest="exec_mode", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=400)
-    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--cpu-steps", type=int, default=8000,
+                    help="oracle steps timed for the CPU baseline (about 10 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (byte-counter calibration)")
@@ -236,6 +255,8 @@ def main():
                     help="only time graph replays of empty kernels of this many blocks")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--batch-nets", type=int, default=256)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched form: keep the default launch shape")
     ap.add_argument("--skip-served", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--skip-large", action="store_true")
@@ -351,7 +372,9 @@ def main():
 
     del sim
     large = None if args.skip_large else bench_large(args, kw)
-    batched = None if args.skip_batched else bench_batched(args)
+    batched = None
+    if not args.skip_batched:
+        batched = {"int": bench_batched(args, True), "float": bench_batched(args, False)}
     served = None
     if not args.skip_served:
         served = {"int": bench_served(args, True), "float": bench_served(args, False)}

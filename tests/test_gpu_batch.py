@@ -82,3 +82,29 @@ def test_batch_float_ring_close_to_alone():
         s.run_steps(steps)
         a, c = b.spikes_total(k), s.spikes_total()
         assert abs(a - c) <= max(3, 0.05 * c), (k, a, c)
+
+
+def test_batch_autotune_keeps_rasters_bit_exact():
+    """The batch autotuner times launch shapes, workgroup sizes and graph
+    lengths from step 0 and resets (stimulus programs included); whatever
+    it picks, every network keeps the raster it gets alone."""
+    steps = 300
+    nets, series = small_batch(steps=steps)
+    cfgs = [cfg_for(k, True, steps) for k in range(len(nets))]
+    b = Batch(nets, cfgs)
+    for k in range(len(nets)):
+        for q in range(8):
+            b.play_input(k, q, series[k][:, q])
+    rep = b.autotune(64)
+    print(rep)
+    assert "chosen" in rep
+    b.run_steps(steps)
+    for k, net in enumerate(nets):
+        _, bc, bi = b.read_spikes(k)
+        s = Sim(net, cfgs[k])
+        for q in range(8):
+            s.play_input(q, series[k][:, q])
+        s.run_steps(steps)
+        _, sc, si = s.read_spikes()
+        s.close()
+        assert np.array_equal(bc, sc) and np.array_equal(bi, si), f"net {k}"
