@@ -214,8 +214,9 @@ def bench_large(args, kw):
     events = sim.events_total() - ev0
     spikes = sim.spikes_total() - sp0
     ev_step = events / steps
-    kbytes = n * B_NEURON + ev_step * B_EVENT
-    kavg_ms = gpu_ms / steps
+    spl = 2 if sim.launch_shape() == 2 else 1     # steps per launch
+    kbytes = spl * (n * B_NEURON + ev_step * B_EVENT)
+    kavg_ms = gpu_ms / (steps / spl)
     achieved = kbytes / (kavg_ms * 1e-3) / 1e9
     traffic = load_traffic(f"traffic_fused_cube{n // 1000}k_{'int' if kw['int_weights'] else 'float'}.txt")
     return {"workload": f"random-cube Izhikevich {n} neurons x Poisson(100) out-synapses, "
@@ -231,7 +232,7 @@ def bench_large(args, kw):
             "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                         "avg_launch_us": round(kavg_ms * 1e3, 3),
+                         "steps_per_launch": spl, "avg_launch_us": round(kavg_ms * 1e3, 3),
                          "algorithmic_bytes_per_launch": int(kbytes)}}
 
 
@@ -317,7 +318,8 @@ def main():
     # never flatters).  Two-kernel shape: a profiled pass with an event pair
     # around every launch splits the step between the two kernels.
     ev_per_step = events / args.steps
-    fused = sim.cfg.step_shape != 0
+    shape = sim.launch_shape()           # 0 two kernels, 1 fused, 2 paired
+    fused = shape != 0
     prof = None
     if args.profile_steps:
         ev1 = sim.events_total()
@@ -326,10 +328,15 @@ def main():
                 "aux_us": round(aux_ms / max(aux_n, 1) * 1e3, 3) if aux_n else None,
                 "note": "direct launches, event pair per launch (adds dispatch latency)"}
         fused = aux_n == 0
+    # steps per launch of the dominant kernel: the paired kernel runs two
+    # (every launch of the timed region is a pair: chunks and the timed
+    # region hold even step counts)
+    spl = 2 if shape == 2 else 1
     if fused:
-        kname = "k_step (fused: update + own-spike push)"
-        kbytes = args.n * B_NEURON + ev_per_step * B_EVENT
-        kavg = gpu_ms / args.steps
+        kname = ("k_pair (two steps per launch: update + own-spike push, delay-1 arrivals pulled)"
+                 if shape == 2 else "k_step (fused: update + own-spike push)")
+        kbytes = spl * (args.n * B_NEURON + ev_per_step * B_EVENT)
+        kavg = gpu_ms / (args.steps / spl)
     elif main_ms >= aux_ms:
         kname = "k_step (update + compaction)"
         kbytes = args.n * B_NEURON
@@ -339,11 +346,12 @@ def main():
         kbytes = ev_per_step * B_EVENT
         kavg = aux_ms / max(aux_n, 1)
     achieved = kbytes / (kavg * 1e-3) / 1e9
-    traffic = load_traffic(f"traffic_{'fused' if fused else 'neuron'}_cube{args.n // 1000}k_{args.ring}.txt")
+    tname = "paired" if shape == 2 else ("fused" if fused else "neuron")
+    traffic = load_traffic(f"traffic_{tname}_cube{args.n // 1000}k_{args.ring}.txt")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
-            "kernel": kname, "avg_launch_us": round(kavg * 1e3, 3),
+            "kernel": kname, "steps_per_launch": spl, "avg_launch_us": round(kavg * 1e3, 3),
             "algorithmic_bytes_per_launch": int(kbytes),
             "bytes_per_neuron_step": B_NEURON, "bytes_per_event": B_EVENT,
             "events_per_step": round(ev_per_step, 1),
@@ -398,7 +406,7 @@ def main():
                                + (" Q16.16" if args.ring == "int" else ""),
                    "neurons": args.n, "synapses": int(len(net["tgt"])),
                    "noise": "gaussian sd 1/step + poisson kicks 1 Hz x 20 mV",
-                   "step_shape": "fused" if fused else "two-kernel",
+                   "step_shape": ["two-kernel", "fused", "paired"][shape],
                    "gen_s": round(t_gen, 2)},
         "sim_ms_per_s": round(args.steps * dt / wall, 1),
         "spikes": int(spikes),

@@ -76,7 +76,13 @@ typedef struct {
                                   nh_read_spikes */
     int32_t exec_mode;         /* 0 auto, 1 direct launches, 2 hipGraph chunks */
     int32_t step_shape;        /* -1 auto, 0 two kernels per step (neuron, scatter),
-                                  1 one fused kernel per step (delay-1 pulled) */
+                                  1 one fused kernel per step (update + own-spike push),
+                                  2 two steps per launch (the delay-1 arrivals of the
+                                  first step pulled by their targets' waves); used where
+                                  the whole grid fits on the device at once and no
+                                  neuron is watched and no synapse has its own tau,
+                                  otherwise runs as 1.  Auto = 2 where at most 256
+                                  delay-1 synapses end in any 64-neuron group, else 1 */
     uint32_t graph_steps;      /* steps per captured graph (0 = 64) */
     uint32_t block;            /* workgroup size of the step kernels, 64..512 (0 = 256) */
 } nh_config;
@@ -186,10 +192,14 @@ int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *mai
                      double *aux_ms, uint64_t *aux_n);
 
 /* Autotuner (addition): from step 0, time every candidate launch shape
- * (one fused kernel or two kernels per step) x workgroup size (64..512) x
- * captured-graph length over trial_steps, keep the fastest and reset the
- * sim.  A text report of the trials goes to report (may be NULL). */
+ * (one fused kernel or two kernels per step, or two steps per launch where
+ * that shape can run) x workgroup size (64..512) x captured-graph length
+ * over trial_steps, keep the fastest and reset the sim.  A text report of
+ * the trials goes to report (may be NULL). */
 int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_cap);
+/* Launch shape the next nh_run_steps uses (addition): 0 two kernels per
+ * step, 1 one fused kernel per step, 2 two steps per launch. */
+int nh_launch_shape(nh_sim *s, int32_t *shape);
 
 /* Measurement helpers (additions, no counterpart in the reference).
  * nh_probe_copy: copy `bytes` between two device buffers with a plain

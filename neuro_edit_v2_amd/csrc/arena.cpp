@@ -285,8 +285,46 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     h_deg_.assign(n_total, 0);
     for (uint32_t i = 0; i < n_total; ++i)
         h_deg_[i] = (src_ptr[i + 1] - src_ptr[i]) + (n_x ? xs_hi[i] - xs_lo[i] : 0);
-    // one kernel per step (own-spike push) unless the two-kernel shape is asked for
-    fused = tune.fused == 0 ? 0 : 1;
+    // one kernel per step (own-spike push) unless the two-kernel or the
+    // paired shape is asked for
+    fused = tune.fused == 0 ? 0 : (tune.fused == 2 ? 2 : 1);
+    // Delay-1 in-tables for the paired shape (asked for, or left to the
+    // autotuner on an arena small enough to be co-resident).  The ring then
+    // gets one slot more than the deepest delay needs, so the pushes of a
+    // paired launch never reach the two slots it reads.
+    pair_tables = (tune.fused == 2 || (tune.fused == -1 && n_total <= (1u << 20))) && n_x == 0 &&
+                  n_total > 0 && n_total < (1u << 26);
+    std::vector<uint32_t> pl_ptr, pl_ent;
+    std::vector<int32_t> pl_wq;
+    std::vector<float> pl_wf;
+    if (pair_tables) {
+        D += 1;
+        // rows are sorted by delay, so a row's delay-1 synapses are its prefix
+        auto tgt_of = [&](uint64_t o) { return packed ? (syn[o] & 0x07FFFFFFu) : syn[o]; };
+        auto d1 = [&](uint64_t o) { return (packed ? syn[o] >> 27 : (uint32_t)d16[o]) == 1u; };
+        pl_ptr.assign((size_t)words + 1, 0);
+        for (uint32_t i = 0; i < n_total; ++i)
+            for (uint64_t o = src_ptr[i]; o < src_ptr[i + 1] && d1(o); ++o) pl_ptr[tgt_of(o) / 64 + 1]++;
+        for (uint32_t w = 0; w < words; ++w) pl_ptr[w + 1] += pl_ptr[w];
+        const uint32_t E1 = pl_ptr[words];
+        pl_ent.assign(std::max<uint32_t>(E1, 1), 0);
+        if (ring_int) pl_wq.assign(std::max<uint32_t>(E1, 1), 0);
+        else pl_wf.assign(std::max<uint32_t>(E1, 1), 0.0f);
+        // auto: paired where the tables are small (at most four pull rounds
+        // of 64 entries per wave); the launcher still checks co-residency
+        uint32_t most = 0;
+        for (uint32_t w = 0; w < words; ++w) most = std::max(most, pl_ptr[w + 1] - pl_ptr[w]);
+        if (tune.fused == -1 && most <= 4u * 64u) fused = 2;
+        std::vector<uint32_t> fill(pl_ptr.begin(), pl_ptr.end() - 1);
+        // entries of a wave in source order
+        for (uint32_t i = 0; i < n_total; ++i)
+            for (uint64_t o = src_ptr[i]; o < src_ptr[i + 1] && d1(o); ++o) {
+                const uint32_t tg = tgt_of(o), e = fill[tg / 64]++;
+                pl_ent[e] = i | ((tg & 63u) << 26);
+                if (ring_int) pl_wq[e] = wq[o];
+                else pl_wf[e] = wf[o];
+            }
+    }
 
     // ---- parameters: type table when the arena has <= 256 distinct rows
     std::map<std::vector<uint32_t>, uint32_t> rows;
@@ -428,6 +466,14 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     if (!packed) TRY(upload(d_delay16, d16));
     if (ring_int) TRY(upload(d_w, wq));
     else TRY(upload(d_w, wf));
+    if (pair_tables) {
+        TRY(upload(d_pl_ptr, pl_ptr));
+        TRY(upload(d_pl_ent, pl_ent));
+        if (ring_int) TRY(upload(d_pl_w, pl_wq));
+        else TRY(upload(d_pl_w, pl_wf));
+        TRY(alloc(d_pub, (size_t)words * 16));
+        TRY(alloc(d_pair_abort, 4));
+    }
     TRY(alloc(d_totals, 8 * 8));
     if (n_x) {
         TRY(upload(d_xt_ptr, xt_ptr));
@@ -553,6 +599,11 @@ StepArgs Arena::args() const {
     a.totals = d_totals.as<uint64_t>();
     a.watch = d_watch.as<uint32_t>();
     a.n_watch = n_watch;
+    a.pl_ptr = d_pl_ptr.as<uint32_t>();
+    a.pl_ent = d_pl_ent.as<uint32_t>();
+    a.pl_w = d_pl_w.p;
+    a.pub = d_pub.as<uint64_t>();
+    a.pair_abort = d_pair_abort.as<uint32_t>();
     a.xrec = d_xrec.as<float2>();
     a.xrec_depth = xrec_depth;
     return a;
@@ -630,9 +681,41 @@ int Arena::check_args(const StepArgs &a) const {
         TRY(need(a.watch, "watch"));
         TRY(need(a.xrec, "xrec"));
     }
+    if (a.pl_ptr) {
+        TRY(need(a.pl_ent, "pl_ent"));
+        TRY(need(a.pl_w, "pl_w"));
+        TRY(need(a.pub, "pub"));
+        TRY(need(a.pair_abort, "pair_abort"));
+    }
     if (a.block % 64 || a.block == 0 || a.block > (uint32_t)kMaxBlock)
         return set_error(NH_ERR_ARG, "bad workgroup size");
     return NH_OK;
+}
+
+// The paired kernel waits on other waves of its own launch, so it runs only
+// when the whole grid fits on the device at once -- here: when it needs at
+// most half the waves the device holds with this workgroup size -- and only
+// for the features it covers (no watched neurons, no custom-tau synapses).
+bool Arena::pair_usable() {
+    if (fused != 2 || !pair_tables || n_watch || n_x || dynamic || words == 0) return false;
+    if (pair_cap_block != block) {
+        pair_cap_waves = pair_resident_waves(ring_int, pmode, mmode, packed != 0, block);
+        pair_cap_block = block;
+    }
+    return (uint64_t)neuron_blocks() * (block / 64) * 2 <= pair_cap_waves;
+}
+
+// A paired launch whose bounded wait ran out leaves the state invalid:
+// report it (once) at the next synchronisation.
+int Arena::check_pair_abort() {
+    if (!pair_launched || !d_pair_abort.p) return NH_OK;
+    pair_launched = false;
+    uint32_t f = 0;
+    HIPCHK(hipMemcpy(&f, d_pair_abort.p, 4, hipMemcpyDeviceToHost));
+    if (!f) return NH_OK;
+    HIPCHK(hipMemset(d_pair_abort.p, 0, 4));
+    return set_error(NH_ERR_HIP, "paired-step launch: a wave's wait for its delay-1 sources ran out "
+                                 "(workgroups not co-resident); the state is invalid, reset the simulation");
 }
 
 // Enqueue k steps starting at arena step t0 (t0 known to the host; the
@@ -641,12 +724,19 @@ int Arena::launch_chunk(uint64_t t0, uint32_t k) {
     StepArgs a = args();
     TRY(check_args(a));
     const uint32_t nb = neuron_blocks(), sb = scat_blocks();
-    for (uint32_t j = 0; j < k; ++j) {
+    const bool pair = pair_usable();
+    for (uint32_t j = 0; j < k;) {
         a.k = j;
         a.slot = (uint32_t)((t0 + j) % D);
+        if (pair && j + 1 < k) {
+            launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, stream);
+            j += 2;
+            continue;
+        }
         launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
         if (!fused) launch_scatter(a, sb, ring_int, packed != 0, stream);
         if (n_watch) launch_watch(a, stream);
+        ++j;
     }
     launch_advance(d_t.as<uint64_t>(), k, stream);
     HIPCHK(hipGetLastError());
@@ -679,10 +769,21 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
         if ((rc = upload_stim(t, (uint32_t)k))) break;
         StepArgs a = args();
         if ((rc = check_args(a))) break;
-        for (uint32_t j = 0; j < k; ++j) {
+        // paired launches count as one main launch of two steps
+        const bool pair = pair_usable();
+        std::vector<uint32_t> firsts;
+        for (uint32_t j = 0; j < k;) {
             a.k = j;
             a.slot = (uint32_t)((t + j) % D);
+            firsts.push_back(j);
             (void)hipEventRecord(ev[4 * j], stream);
+            if (pair && j + 1 < k) {
+                launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, stream);
+                (void)hipEventRecord(ev[4 * j + 1], stream);
+                pair_launched = true;
+                j += 2;
+                continue;
+            }
             launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
             (void)hipEventRecord(ev[4 * j + 1], stream);
             if (!fused) {
@@ -691,6 +792,7 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
                 (void)hipEventRecord(ev[4 * j + 3], stream);
             }
             if (n_watch) launch_watch(a, stream);
+            ++j;
         }
         launch_advance(d_t.as<uint64_t>(), (uint32_t)k, stream);
         hipError_t e = hipStreamSynchronize(stream);
@@ -698,7 +800,8 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
             rc = set_error(NH_ERR_HIP, std::string("profile: ") + hipGetErrorString(e));
             break;
         }
-        for (uint32_t j = 0; j < k; ++j) {
+        if ((rc = check_pair_abort())) break;
+        for (uint32_t j : firsts) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, ev[4 * j], ev[4 * j + 1]) == hipSuccess) {
                 main_ms += ms;
@@ -814,6 +917,7 @@ int Arena::run(uint64_t n_steps) {
         TRY(upload_stim(t, (uint32_t)k));
         if (graph_ok && k == graph_steps) HIPCHK(hipGraphLaunch(graph_exec, stream));
         else TRY(launch_chunk(t, (uint32_t)k));
+        if (k > 1 && pair_usable()) pair_launched = true;
         t += k;
         n_steps -= k;
     }
@@ -829,11 +933,13 @@ int Arena::sync() {
         if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_run_ms = ms;
         timing_valid = false;
     }
-    return NH_OK;
+    return check_pair_abort();
 }
 
 int Arena::reset() {
-    TRY(sync());
+    // (a failed paired launch is what a reset recovers from: no abort check)
+    HIPCHK(hipStreamSynchronize(stream));
+    timing_valid = false;
     HIPCHK(hipMemcpyAsync(d_x.p, d_x0.p, d_x0.bytes, hipMemcpyDeviceToDevice, stream));
     HIPCHK(hipMemsetAsync(d_s.p, 0, d_s.bytes, stream));
     HIPCHK(hipMemsetAsync(d_ring.p, 0, d_ring.bytes, stream));
@@ -849,13 +955,22 @@ int Arena::reset() {
     HIPCHK(hipMemsetAsync(d_net_spikes.p, 0, d_net_spikes.bytes, stream));
     HIPCHK(hipMemsetAsync(d_rast_ctl.p, 0, d_rast_ctl.bytes, stream));
     HIPCHK(hipMemsetAsync(d_t.p, 0, 8, stream));
+    if (d_pub.p) {
+        HIPCHK(hipMemsetAsync(d_pub.p, 0, d_pub.bytes, stream));
+        HIPCHK(hipMemsetAsync(d_pair_abort.p, 0, d_pair_abort.bytes, stream));
+    }
     HIPCHK(hipStreamSynchronize(stream));
+    pair_launched = false;
     t = 0;
     rast_step0 = 0;
     h_rast_cnt.clear();
     h_rast_ids.clear();
     h_rast_step0 = 0;
     return NH_OK;
+}
+
+static std::string shape_name(int f) {
+    return f == 2 ? "paired" : (f ? "fused" : "two-kernel");
 }
 
 int Arena::autotune(uint32_t trial_steps, std::string &report) {
@@ -867,10 +982,21 @@ int Arena::autotune(uint32_t trial_steps, std::string &report) {
         double ms;
     };
     std::vector<Cand> cands;
-    for (int f : {1, 0})
+    for (int f : {1, 0, 2})
         for (uint32_t b : {64u, 128u, 256u, 512u})
             for (uint32_t g : {64u, 256u}) {
                 if (!use_graph && g != 64u) continue;
+                if (f == 2) {
+                    // only where a paired launch would actually run
+                    const int keep_f = fused;
+                    const uint32_t keep_b = block;
+                    fused = 2;
+                    block = b;
+                    const bool ok = pair_usable();
+                    fused = keep_f;
+                    block = keep_b;
+                    if (!ok) continue;
+                }
                 cands.push_back({f, b, g, 0.0});
             }
     Cand best{fused, block, graph_steps_req, 1e30};
@@ -887,7 +1013,7 @@ int Arena::autotune(uint32_t trial_steps, std::string &report) {
         TRY(run(trial_steps));
         TRY(sync());
         c.ms = last_run_ms;
-        report += (c.fused ? "fused" : "two-kernel") + std::string(" block ") +
+        report += shape_name(c.fused) + std::string(" block ") +
                   std::to_string(c.block) + " graph " + std::to_string(graph_steps) + ": " +
                   std::to_string(c.ms * 1e3 / trial_steps) + " us/step\n";
         if (c.ms < best.ms) best = c;
@@ -897,7 +1023,7 @@ int Arena::autotune(uint32_t trial_steps, std::string &report) {
     graph_steps_req = best.graph;
     graph_dirty = true;
     TRY(reset());
-    report += "chosen: " + std::string(best.fused ? "fused" : "two-kernel") + " block " +
+    report += "chosen: " + shape_name(best.fused) + " block " +
               std::to_string(best.block) + " graph " + std::to_string(best.graph) + "\n";
     return NH_OK;
 }

@@ -21,6 +21,9 @@ void launch_watch(const StepArgs &a, hipStream_t st);
 void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
 void launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t lane_bytes, hipStream_t st);
 void launch_empty(uint32_t blocks, hipStream_t st);
+void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
+                 hipStream_t st);
+uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, uint32_t block);
 
 int set_error(int code, const std::string &msg);
 
@@ -59,7 +62,7 @@ struct NetSlot {
 };
 
 struct Tuning {
-    int fused = -1;          // -1 auto, 0 two-phase, 1 fused
+    int fused = -1;          // -1 auto, 0 two-phase, 1 fused, 2 paired (two steps per launch)
     int use_graph = -1;      // -1 auto
     uint32_t graph_steps = 64;
     uint32_t scatter_blocks = 0;  // 0 auto
@@ -137,6 +140,16 @@ class Arena {
     // custom-tau synapses
     DBuf d_xt_ptr, d_xt_idx, d_xdecay, d_xsyn, d_xring, d_xs_lo, d_xs_hi, d_xs_k, d_xs_d, d_xs_w;
     uint32_t n_x = 0;
+    // paired-step shape: delay-1 in-tables per wave, published spike words,
+    // wait-abort flag; built when the shape may be used (the ring then has
+    // one slot more than the deepest delay needs)
+    DBuf d_pl_ptr, d_pl_ent, d_pl_w, d_pub, d_pair_abort;
+    bool pair_tables = false;
+    bool pair_launched = false;          // paired launches since the last abort check
+    uint32_t pair_cap_block = 0, pair_cap_waves = 0;
+    // true when the next chunk may use the paired kernel
+    bool pair_usable();
+    int check_pair_abort();
     std::vector<uint32_t> h_deg_;      // total out-degree per arena neuron
     uint32_t n_win = 0;
     bool dynamic = false;      // served arena: slots come and go

@@ -122,8 +122,9 @@ __device__ __forceinline__ uint64_t ring_index(const StepArgs &a, uint32_t sy, u
 // atomics.  With a ring below 2 GB the atomics are buffer atomics whose
 // empty slots carry an out-of-range offset, so the batch has no branch
 // per slot (branches there make the compiler wait for each atomic before
-// the next); otherwise each slot's atomic is guarded.
-template <int RING, bool PACKED>
+// the next); otherwise each slot's atomic is guarded.  SKIP1 drops the
+// delay-1 entries (the paired-step kernel pulls those instead).
+template <int RING, bool PACKED, bool SKIP1 = false>
 __device__ __forceinline__ void deliver_batch(const StepArgs &a, const uint32_t (&j)[kPushUnroll],
                                               uint32_t slot0) {
     using RT = typename Ring<RING>::T;
@@ -137,18 +138,22 @@ __device__ __forceinline__ void deliver_batch(const StepArgs &a, const uint32_t 
         wv[r] = w[jr];
         d16[r] = PACKED ? 0u : a.delay16[jr];
     }
+    bool keep[kPushUnroll];
+#pragma unroll
+    for (uint32_t r = 0; r < kPushUnroll; ++r)
+        keep[r] = j[r] != kNoEntry && (!SKIP1 || (PACKED ? sy[r] >> 27 : d16[r]) != 1u);
     if (a.ring_buf) {
         const i32x4 rs = buffer_rsrc(a.ring, a.ring_bytes);
 #pragma unroll
         for (uint32_t r = 0; r < kPushUnroll; ++r) {
             // ring below 2 GB: the byte offset fits in 31 bits
             const uint32_t off = (uint32_t)ring_index<PACKED>(a, sy[r], d16[r], slot0) * 4u;
-            Ring<RING>::add_buf(rs, j[r] != kNoEntry ? off : kBufDrop, wv[r]);
+            Ring<RING>::add_buf(rs, keep[r] ? off : kBufDrop, wv[r]);
         }
     } else {
 #pragma unroll
         for (uint32_t r = 0; r < kPushUnroll; ++r)
-            if (j[r] != kNoEntry)
+            if (keep[r])
                 Ring<RING>::add((RT *)a.ring + ring_index<PACKED>(a, sy[r], d16[r], slot0), wv[r]);
     }
 }
@@ -174,7 +179,7 @@ __device__ __forceinline__ void push_row(const StepArgs &a, uint32_t lo, uint32_
 // concatenated in lane order and the wave walks the list kPushUnroll*64
 // entries at a time, so a wave whose spikes own few synapses pays one load
 // round trip in total instead of one per spike and per 64 entries.
-template <int RING, bool PACKED>
+template <int RING, bool PACKED, bool SKIP1 = false>
 __device__ __forceinline__ void push_wave(const StepArgs &a, uint64_t mask, uint32_t row_lo,
                                           uint32_t row_hi, uint32_t slot0, uint32_t lane) {
     constexpr uint32_t kSpan = kPushUnroll * kWave;
@@ -211,7 +216,7 @@ __device__ __forceinline__ void push_wave(const StepArgs &a, uint64_t mask, uint
             }
             s0 += d;
         }
-        deliver_batch<RING, PACKED>(a, j, slot0);
+        deliver_batch<RING, PACKED, SKIP1>(a, j, slot0);
     }
 }
 
@@ -228,6 +233,120 @@ __device__ __forceinline__ void push_custom(const StepArgs &a, uint32_t src, uin
         uint32_t sl = slot0 + a.xs_d[e];
         sl = sl >= a.D ? sl - a.D : sl;
         Ring<RING>::add(xr + (size_t)sl * a.n_x + a.xs_k[e], w[e]);
+    }
+}
+
+// One neuron's update at step t, in the order of sim/src/cpu.rs:217-260:
+// ring arrival rv, custom-tau traces (ring slot `slot`), stimulus (window
+// row kk, skipped when stim_on is false), Poisson kick, Gaussian current,
+// model step.  s holds the synaptic current before the step and returns it
+// decayed; xs is the model state.  Returns whether the neuron spiked.
+template <int RING, int PMODE, int MMODE>
+__device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &nr, uint32_t i,
+                                              uint32_t local, uint64_t tl, uint32_t slot,
+                                              uint32_t kk, bool stim_on, uint32_t prow,
+                                              typename Ring<RING>::T rv, float ext, float &s,
+                                              float2 &xs) {
+    using RT = typename Ring<RING>::T;
+    bool spiked;
+    const RingScale rsc(nr.int_scale);
+    const float arr = Ring<RING>::get(rv, rsc);
+    const float sv = s + arr;
+    float I = sv + ext;
+    if (a.xt_ptr) {
+        // custom-tau traces into this neuron (cpu.rs:208-228): this
+        // step's arrival, use, then decay -- each slot has one target,
+        // so its thread owns it
+        RT *xr = (RT *)a.xring + (size_t)slot * a.n_x;
+        for (uint32_t j = a.xt_ptr[i]; j < a.xt_ptr[i + 1]; ++j) {
+            const uint32_t kx = a.xt_idx[j];
+            const RT av = xr[kx];
+            xr[kx] = (RT)0;
+            const float xv = a.xsyn[kx] + Ring<RING>::get(av, rsc);
+            I += xv;
+            a.xsyn[kx] = xv * a.xdecay[kx];
+        }
+    }
+    if (stim_on) {
+        // Stimulus (stim.rs:119-127).  Skipped when every input of the
+        // network is zero this step: each term is then w * 0 = +-0, and
+        // I + (+-0) == I because I is never -0 here (ring sums start
+        // from +0 and never reach -0, so s + arrival is not -0, nor is
+        // that plus ext or plus a custom-tau trace).
+        const float *row = a.stim_win + (size_t)kk * a.stim_cols + nr.stim_base;
+        for (uint32_t j = a.stim_lo[i]; j < a.stim_hi[i]; ++j)
+            I += a.stim_w[j] * row[a.stim_input[j]];
+    }
+    float pr[kMaxParams];
+    uint32_t model = (uint32_t)MMODE;
+    if (PMODE == 0) {
+#pragma unroll
+        for (int q = 0; q < kMaxParams; ++q) pr[q] = a.krow[0].p[q];
+        model = a.krow[0].model;
+#pragma unroll
+        for (int r = 1; r < kKargRows; ++r) {
+            if ((uint32_t)r >= a.n_krows) break;  // uniform: only the rows in use
+            const bool sel = prow == (uint32_t)r;
+#pragma unroll
+            for (int q = 0; q < kMaxParams; ++q) pr[q] = sel ? a.krow[r].p[q] : pr[q];
+            model = sel ? a.krow[r].model : model;
+        }
+    } else if (PMODE == 2) {
+        const PRow &r = a.ptab[prow];
+#pragma unroll
+        for (int q = 0; q < kMaxParams; ++q) pr[q] = r.p[q];
+        model = r.model;
+    } else {
+        model = MMODE == 2 ? a.mreg[i] : (uint32_t)MMODE;
+        if (MMODE == 0 && a.P == 4) {
+            const float4 v = *(const float4 *)(a.p + (size_t)i * 4);
+            pr[0] = v.x; pr[1] = v.y; pr[2] = v.z; pr[3] = v.w;
+        } else {
+            for (uint32_t q = 0; q < a.P && q < (uint32_t)kMaxParams; ++q)
+                pr[q] = a.p[(size_t)i * a.P + q];
+        }
+    }
+    const bool is_adex = MMODE == 1 || (MMODE == 2 && model == 1);
+    // sim/src/cpu.rs:240-245: kick * model gain (C for AdEx, adex.rs:189-191)
+    const float gain = is_adex ? pr[0] : 1.0f;
+    if (nr.pkick > 0.0f || nr.sd != 0.0f) {
+        // one hash serves the kick (high half) and the Gaussian (low half)
+        const uint64_t key = noise_key(local, tl, nr.seed);
+        const uint64_t h0 = mix64(key);
+        if (nr.pkick > 0.0f && kick_fires(h0, nr.pkick)) I += nr.kick * gain;
+        if (nr.sd != 0.0f) I += (nr.sd * gauss_draw(h0, key)) * gain;
+    }
+    if (is_adex)
+        spiked = adex_step(pr, xs, I, nr.dt);
+    else
+        spiked = izh_step(pr[0], pr[1], pr[2], pr[3], xs, I, nr.dt);
+    s = sv * nr.decay_s;
+    return spiked;
+}
+
+// Per-neuron marks of a spike at step t (local step tl): lifetime count,
+// last spike step, window counter.  No-return atomics: nothing waits on a
+// read of the old count (each neuron owns its counter, so no contention).
+__device__ __forceinline__ void spike_marks(const StepArgs &a, const NetRec &nr, uint32_t i,
+                                            uint64_t t, uint64_t tl) {
+    atomicAdd(&a.pn_total[i], 1u);
+    a.last_spike[i] = t < 0xFFFFFFFEull ? (uint32_t)t : 0xFFFFFFFEu;
+    if (a.wcnt && nr.win_steps) {
+        const uint64_t wi = tl / nr.win_steps;
+        if (wi < a.n_win) atomicAdd(&a.wcnt[(size_t)wi * a.n_total + i], 1u);
+    }
+}
+
+// The wave's spike word of step t into the record ring and the raster
+// window (lane 0 only).  The host drains the raster window before it
+// fills; the flag reports a step outside it.
+__device__ __forceinline__ void spike_words(const StepArgs &a, uint32_t wave, uint64_t t,
+                                            uint64_t rast_origin, uint64_t mask) {
+    if (a.bits) a.bits[(size_t)(t % a.bits_rows) * a.words + wave] = mask;
+    if (a.rast_bits) {
+        const uint64_t rs = t - rast_origin;
+        if (rs < a.rast_cap_steps) a.rast_bits[(size_t)rs * a.words + wave] = mask;
+        else a.rast_ctl[2] = 1;
     }
 }
 
@@ -285,80 +404,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     const bool act = local < nr.n && nr.active && (nr.t_len == 0 || tl < nr.t_len);
     bool spiked = false;
     float s_new = s_old;
-    if (act) {
-        const RingScale rsc(nr.int_scale);
-        const float arr = Ring<RING>::get(rv, rsc);
-        const float sv = s_old + arr;
-        float I = sv + ext;
-        if (a.xt_ptr) {
-            // custom-tau traces into this neuron (cpu.rs:208-228): this
-            // step's arrival, use, then decay -- each slot has one target,
-            // so its thread owns it
-            RT *xr = (RT *)a.xring + (size_t)slot * a.n_x;
-            for (uint32_t j = a.xt_ptr[i]; j < a.xt_ptr[i + 1]; ++j) {
-                const uint32_t kx = a.xt_idx[j];
-                const RT av = xr[kx];
-                xr[kx] = (RT)0;
-                const float xv = a.xsyn[kx] + Ring<RING>::get(av, rsc);
-                I += xv;
-                a.xsyn[kx] = xv * a.xdecay[kx];
-            }
-        }
-        if (stim_on) {
-            // Stimulus (stim.rs:119-127).  Skipped when every input of the
-            // network is zero this step: each term is then w * 0 = +-0, and
-            // I + (+-0) == I because I is never -0 here (ring sums start
-            // from +0 and never reach -0, so s + arrival is not -0, nor is
-            // that plus ext or plus a custom-tau trace).
-            const float *row = a.stim_win + (size_t)a.k * a.stim_cols + nr.stim_base;
-            for (uint32_t j = a.stim_lo[i]; j < a.stim_hi[i]; ++j)
-                I += a.stim_w[j] * row[a.stim_input[j]];
-        }
-        float pr[kMaxParams];
-        uint32_t model = (uint32_t)MMODE;
-        if (PMODE == 0) {
-#pragma unroll
-            for (int q = 0; q < kMaxParams; ++q) pr[q] = a.krow[0].p[q];
-            model = a.krow[0].model;
-#pragma unroll
-            for (int r = 1; r < kKargRows; ++r) {
-                if ((uint32_t)r >= a.n_krows) break;  // uniform: only the rows in use
-                const bool sel = prow == (uint32_t)r;
-#pragma unroll
-                for (int q = 0; q < kMaxParams; ++q) pr[q] = sel ? a.krow[r].p[q] : pr[q];
-                model = sel ? a.krow[r].model : model;
-            }
-        } else if (PMODE == 2) {
-            const PRow &r = a.ptab[prow];
-#pragma unroll
-            for (int q = 0; q < kMaxParams; ++q) pr[q] = r.p[q];
-            model = r.model;
-        } else {
-            model = MMODE == 2 ? a.mreg[i] : (uint32_t)MMODE;
-            if (MMODE == 0 && a.P == 4) {
-                const float4 v = *(const float4 *)(a.p + (size_t)i * 4);
-                pr[0] = v.x; pr[1] = v.y; pr[2] = v.z; pr[3] = v.w;
-            } else {
-                for (uint32_t q = 0; q < a.P && q < (uint32_t)kMaxParams; ++q)
-                    pr[q] = a.p[(size_t)i * a.P + q];
-            }
-        }
-        const bool is_adex = MMODE == 1 || (MMODE == 2 && model == 1);
-        // sim/src/cpu.rs:240-245: kick * model gain (C for AdEx, adex.rs:189-191)
-        const float gain = is_adex ? pr[0] : 1.0f;
-        if (nr.pkick > 0.0f || nr.sd != 0.0f) {
-            // one hash serves the kick (high half) and the Gaussian (low half)
-            const uint64_t key = noise_key(local, tl, nr.seed);
-            const uint64_t h0 = mix64(key);
-            if (nr.pkick > 0.0f && kick_fires(h0, nr.pkick)) I += nr.kick * gain;
-            if (nr.sd != 0.0f) I += (nr.sd * gauss_draw(h0, key)) * gain;
-        }
-        if (is_adex)
-            spiked = adex_step(pr, xs, I, nr.dt);
-        else
-            spiked = izh_step(pr[0], pr[1], pr[2], pr[3], xs, I, nr.dt);
-        s_new = sv * nr.decay_s;
-    }
+    if (act)
+        spiked = neuron_update<RING, PMODE, MMODE>(a, nr, i, local, tl, slot, a.k, stim_on, prow, rv,
+                                                   ext, s_new, xs);
 
     // ---- wave64 ballot: the mask IS this wave's word of the spike bitmap
     const uint64_t mask = __ballot(spiked);
@@ -382,27 +430,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
         ring[ri] = (RT)0;
         a.x[i] = xs;
         a.s[i] = s_new;
-        if (spiked) {
-            // no-return atomics: nothing waits on a read of the old count
-            // (each neuron owns its counter, so there is no contention)
-            atomicAdd(&a.pn_total[i], 1u);
-            a.last_spike[i] = t < 0xFFFFFFFEull ? (uint32_t)t : 0xFFFFFFFEu;
-            if (a.wcnt && nr.win_steps) {
-                const uint64_t wi = tl / nr.win_steps;
-                if (wi < a.n_win) atomicAdd(&a.wcnt[(size_t)wi * a.n_total + i], 1u);
-            }
-        }
+        if (spiked) spike_marks(a, nr, i, t, tl);
     }
-    if (lane == 0) {
-        if (a.bits) a.bits[(size_t)(t % a.bits_rows) * a.words + wave] = mask;
-        if (a.rast_bits) {
-            // raster window row of this step; the host drains the window
-            // before it fills, the flag reports a step outside it
-            const uint64_t rs = t - rast_origin;
-            if (rs < a.rast_cap_steps) a.rast_bits[(size_t)rs * a.words + wave] = mask;
-            else a.rast_ctl[2] = 1;
-        }
-    }
+    if (lane == 0) spike_words(a, wave, t, rast_origin, mask);
     const uint32_t L = (uint32_t)(t % 3);
     // two-kernel shape: the list of step t+1 starts empty (its appends come
     // in the next launch, after this one has finished)
@@ -418,6 +448,158 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
         off = uniform(off);
         if (spiked) a.spk[(size_t)L * a.n_total + off + wave_rank(mask)] = i;
     }
+}
+
+// ---------------------------------------------------------------- pair
+// Two steps per launch, t0 = t and t1 = t + 1.  Between the two steps the
+// only dependency that crosses waves is a delay-1 synapse: a spike at t0
+// arrives at t1.  Those arrivals are not pushed; the target's wave pulls
+// them.  Each wave publishes its t0 spike word (two 64-bit words, each
+// holding a 32-bit half and the tag t0+1, stored whole), then reads the
+// words of the sources in its delay-1 in-table (source, target lane,
+// weight) until they carry the tag, and adds the weights of the sources
+// that fired -- a point-to-point wait on the waves that feed it, no grid
+// barrier.  Every other delivery of t0 is pushed as in k_step; the ring
+// has one slot more than the deepest delay needs, so the pushes of this
+// launch (slots t0+2 .. t0+1+maxdelay) never touch slots t0 and t0+1,
+// which the launch reads and clears.  (v, u) and s stay in registers
+// between the two steps.  With the int ring the pulled weights are the
+// same integers the ring would have summed, so the result is the one-step
+// result bit for bit.
+//
+// The wait needs every source wave to run while its consumers wait, so the
+// launcher uses this kernel only when the whole grid fits on the device at
+// once; the wait is bounded anyway: past kPairSpins polls a wave raises
+// pair_abort (the host reports the launch as failed) and moves on, and
+// every other waiting wave sees the flag and stops waiting.
+constexpr uint32_t kPairSpins = 1u << 16;
+
+template <typename RT>
+__device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t ent, RT w,
+                                        uint32_t tag, RT *acc_w, uint32_t lane) {
+    const uint32_t src = ent & 0x03FFFFFFu;
+    const uint64_t *pw = a.pub + 2u * (src >> 6) + ((src >> 5) & 1u);
+    uint64_t v = 0;
+    for (uint32_t spins = 0;; ++spins) {
+        if (has) v = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool ok = !has || (uint32_t)(v >> 32) == tag;
+        if (__ballot(!ok) == 0ull) break;
+        if (spins >= kPairSpins ||
+            ((spins & 63u) == 63u &&
+             __hip_atomic_load(a.pair_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            if (lane == 0)
+                __hip_atomic_store(a.pair_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (has && (((uint32_t)v >> (src & 31u)) & 1u)) atomicAdd(&acc_w[ent >> 26], w);
+}
+
+template <int RING, int PMODE, int MMODE, bool PACKED>
+__global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
+    using RT = typename Ring<RING>::T;
+    __shared__ RT acc[kMaxBlock];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
+    if (wave >= a.words) return;
+    const uint32_t i = wave * kWave + lane;
+    const uint32_t slot0 = a.slot;
+    const uint32_t slot1 = slot0 + 1u == a.D ? 0u : slot0 + 1u;
+    NetRec nr;
+    uint32_t net = 0;
+    if (a.single_net) {
+        nr = a.net0;
+    } else {
+        net = uniform((uint32_t)a.wave_net[wave]);
+        nr = a.nets[net];
+    }
+
+    // ---- every load of the launch up front
+    RT *ring = (RT *)a.ring;
+    const size_t r0 = (size_t)slot0 * a.n_total + i, r1 = (size_t)slot1 * a.n_total + i;
+    const RT rv0 = ring[r0];
+    const RT rv1 = ring[r1];
+    float s = a.s[i];
+    float2 xs = a.x[i];
+    const float ext = a.ext ? a.ext[i] : 0.0f;
+    const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
+    uint32_t prow = 0;
+    if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
+    const uint64_t t0 = *a.t_dev + a.k;
+    const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
+    const bool stim0 = a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
+    const bool stim1 = a.stim_lo && a.stim_nz[(size_t)(a.k + 1) * a.stim_groups + net] != 0u;
+    // this wave's delay-1 in-table; the first 64 entries are loaded here
+    const uint32_t pl_lo = a.pl_ptr[wave], pl_hi = a.pl_ptr[wave + 1];
+    const RT *pl_w = (const RT *)a.pl_w;
+    const bool has0 = pl_lo + lane < pl_hi;
+    uint32_t ent0 = 0;
+    RT w0 = (RT)0;
+    if (has0) {
+        ent0 = a.pl_ent[pl_lo + lane];
+        w0 = pl_w[pl_lo + lane];
+    }
+
+    // ---- step t0
+    const uint32_t local = i - nr.base;
+    const uint64_t tl0 = t0 - (uint64_t)nr.t0;
+    const bool act0 = local < nr.n && nr.active && (nr.t_len == 0 || tl0 < nr.t_len);
+    bool sp0 = false;
+    if (act0)
+        sp0 = neuron_update<RING, PMODE, MMODE>(a, nr, i, local, tl0, slot0, a.k, stim0, prow, rv0,
+                                                ext, s, xs);
+    const uint64_t m0 = __ballot(sp0);
+    const uint32_t tag = (uint32_t)t0 + 1u;   // never 0 before 2^32 steps: 0 is "unwritten"
+    if (lane < 2u) {
+        const uint32_t half = lane ? (uint32_t)(m0 >> 32) : (uint32_t)m0;
+        __hip_atomic_store(&a.pub[2u * wave + lane], ((uint64_t)tag << 32) | half, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // The push and stores of t0 go between the publication and the poll:
+    // another XCD sees a published word only about two memory round trips
+    // after it is stored, and this work fills that time (done after the
+    // poll it would add its own round trips to the end of the launch).
+    if (m0) push_wave<RING, PACKED, true>(a, m0, row_lo, row_hi, slot0, lane);
+    if (sp0) spike_marks(a, nr, i, t0, tl0);
+    if (lane == 0) spike_words(a, wave, t0, rast_origin, m0);
+
+    // ---- pull the delay-1 arrivals of t0 into this wave's neurons
+    RT *acc_w = acc + (threadIdx.x & ~(uint32_t)(kWave - 1));
+    acc_w[lane] = (RT)0;
+    pull_d1<RT>(a, has0, ent0, w0, tag, acc_w, lane);
+    for (uint32_t b = pl_lo + kWave; b < pl_hi; b += kWave) {
+        const bool has = b + lane < pl_hi;
+        uint32_t ent = 0;
+        RT w = (RT)0;
+        if (has) {
+            ent = a.pl_ent[b + lane];
+            w = pl_w[b + lane];
+        }
+        pull_d1<RT>(a, has, ent, w, tag, acc_w, lane);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const RT rv1p = rv1 + acc_w[lane];
+
+    // ---- step t1
+    const uint64_t t1 = t0 + 1u, tl1 = tl0 + 1u;
+    const bool act1 = local < nr.n && nr.active && (nr.t_len == 0 || tl1 < nr.t_len);
+    bool sp1 = false;
+    if (act1)
+        sp1 = neuron_update<RING, PMODE, MMODE>(a, nr, i, local, tl1, slot1, a.k + 1u, stim1, prow,
+                                                rv1p, ext, s, xs);
+    const uint64_t m1 = __ballot(sp1);
+    if (m1) push_wave<RING, PACKED, false>(a, m1, row_lo, row_hi, slot1, lane);
+
+    // ---- stores of both steps
+    if (act0) ring[r0] = (RT)0;
+    if (act1) ring[r1] = (RT)0;
+    if (act0 || act1) {
+        a.x[i] = xs;
+        a.s[i] = s;
+    }
+    if (sp1) spike_marks(a, nr, i, t1, tl1);
+    if (lane == 0) spike_words(a, wave, t1, rast_origin, m1);
 }
 
 // ---------------------------------------------------------------- scatter
@@ -531,6 +713,50 @@ void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mm
                  bool packed, hipStream_t st) {
     if (ring == 0) launch_r<0>(a, blocks, pmode, mmode, push, packed, st);
     else launch_r<1>(a, blocks, pmode, mmode, push, packed, st);
+}
+
+template <int RING, int PMODE, int MMODE>
+static const void *pair_fn(bool packed) {
+    return packed ? (const void *)k_pair<RING, PMODE, MMODE, true>
+                  : (const void *)k_pair<RING, PMODE, MMODE, false>;
+}
+
+static const void *pair_kernel(int ring, int pmode, int mmode, bool packed) {
+#define NH_PAIR_M(R, P)                                                                           \
+    switch (mmode) {                                                                              \
+    case 0: return pair_fn<R, P, 0>(packed);                                                      \
+    case 1: return pair_fn<R, P, 1>(packed);                                                      \
+    default: return pair_fn<R, P, 2>(packed);                                                     \
+    }
+#define NH_PAIR_P(R)                                                                              \
+    switch (pmode) {                                                                              \
+    case 0: NH_PAIR_M(R, 0)                                                                       \
+    case 1: NH_PAIR_M(R, 1)                                                                       \
+    default: NH_PAIR_M(R, 2)                                                                      \
+    }
+    if (ring == 0) { NH_PAIR_P(0) }
+    NH_PAIR_P(1)
+#undef NH_PAIR_P
+#undef NH_PAIR_M
+}
+
+void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
+                 hipStream_t st) {
+    void *args[] = {(void *)&a};
+    (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed), dim3(blocks), dim3(a.block), args,
+                          0, st);
+}
+
+// Waves of the paired-step kernel that the device holds at once with
+// workgroups of `block` threads (0 when the query fails).
+uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, uint32_t block) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair_kernel(ring, pmode, mmode, packed),
+                                                     (int)block, 0) != hipSuccess)
+        return 0;
+    return (uint32_t)cus * (uint32_t)per_cu * (block / 64u);
 }
 
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st) {

@@ -204,6 +204,13 @@ int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *mai
     return NH_OK;
 }
 
+int nh_launch_shape(nh_sim *s, int32_t *shape) {
+    CHECK_SIM(s);
+    if (!shape) return set_error(NH_ERR_ARG, "null output");
+    *shape = s->ar.pair_usable() ? 2 : (s->ar.fused ? 1 : 0);
+    return NH_OK;
+}
+
 int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_cap) {
     CHECK_SIM(s);
     std::string rep;

@@ -254,6 +254,13 @@ class Sim:
         L.check(L.lib().nh_autotune(self._h, trial_steps, buf, 4096))
         return buf.value.decode()
 
+    def launch_shape(self):
+        """Launch shape of the next run: 0 two kernels per step, 1 one fused
+        kernel per step, 2 two steps per launch."""
+        v = C.c_int32()
+        L.check(L.lib().nh_launch_shape(self._h, C.byref(v)))
+        return v.value
+
     def last_run_ms(self):
         v = C.c_double()
         L.check(L.lib().nh_last_run_ms(self._h, C.byref(v)))

@@ -1,0 +1,183 @@
+"""Two steps per launch (step_shape=2): the delay-1 arrivals of the first
+step of a launch are pulled by their targets' waves instead of pushed.
+
+Int ring: raster, v, u and I are the oracle's bit for bit whatever the mix
+of delays -- few delay-1 synapses (random cube), many (grid liquid with
+local wiring, stimulus inputs, Izhikevich and AdEx), all of them, with more
+than 64 delay-1 synapses into one wave -- over odd step counts (a one-step
+launch closes each chunk), with graph and direct launches.  Float ring:
+within the tolerances of the fused shape.  Where the shape cannot run
+(watched neurons) the step falls back to one launch per step.
+"""
+import numpy as np
+import pytest
+
+from neuro_edit_v2_amd import Sim, SimConfig
+from neuro_edit_v2_amd import net as N
+from neuro_edit_v2_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CFG0 = dict(dt_ms=0.5, synapse_tau_ms=5.0, noise_sd=1.0, noise_rate_hz=20.0, noise_kick_mv=20.0,
+            seed=7)
+
+
+def all_delay1_net(n, deg, seed):
+    """Random targets, every synapse of delay 1 (the reference's default
+    delay at dt = delay_ms), signed weights."""
+    r = np.random.default_rng(seed)
+    E = n * deg
+    w = r.uniform(0.2, 2.5, E).astype(np.float32)
+    w[r.random(E) < 0.2] *= -1.0
+    return dict(n=n, src_ptr=np.arange(0, E + 1, deg, dtype=np.uint32),
+                tgt=r.integers(0, n, E, dtype=np.uint32), w=w,
+                delay=np.ones(E, np.uint16),
+                model_ids=np.zeros(1, np.uint8), model=np.zeros(n, np.uint8), n_params=4,
+                n_state=2, p=np.tile(np.float32([0.02, 0.2, -65.0, 8.0]), n),
+                x0=np.tile(np.float32([-65.0, -13.0]), n))
+
+
+def delay1_per_wave(net):
+    d1 = net["delay"] == 1
+    return np.bincount(net["tgt"][d1] // 64, minlength=(net["n"] + 63) // 64)
+
+
+def grid_case(model):
+    net = N.grid_net(0, model=model, seed=3 + model, dt_ms=0.5, dims=(17, 16, 15))
+    bits = W.stream_window(W.rng_stream(50 + model, 8, 0.05), 8, 301)
+    return net, bits
+
+
+def check_int(net, steps, kw, series=None, exec_mode=2, exp_mode=0):
+    sim = Sim(net, SimConfig(raster_steps=steps, step_shape=2, exec_mode=exec_mode, **kw))
+    assert sim.launch_shape() == 2
+    ref = O.OracleSim(net, O.default_cfg(exp_mode=exp_mode, **kw))
+    if series is not None:
+        for q in range(series.shape[1]):
+            sim.play_input(q, series[:, q])
+            ref.play_input(q, series[:, q])
+    sim.run_steps(steps)
+    _, cnt, ids = sim.read_spikes()
+    rc, ri = ref.run_steps(steps, record=True)
+    print(f"spikes gpu={cnt.sum()} oracle={rc.sum()}")
+    assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
+    assert np.array_equal(sim.v(), ref.v().astype(np.float32))
+    assert np.array_equal(sim.state(1), ref.state(1).astype(np.float32))
+    assert np.array_equal(sim.syn_current(), ref.syn_current().astype(np.float32))
+    assert np.array_equal(sim.pn_total(), ref.pn_total())
+    return int(rc.sum())
+
+
+@pytest.mark.parametrize("exec_mode", [1, 2], ids=["direct", "graph"])
+def test_paired_cube_int_bit_exact(exec_mode):
+    net = N.cube_net(10000, mean_deg=100.0, seed=1, w_scale=0.7)
+    assert delay1_per_wave(net).sum() > 0
+    kw = dict(CFG0, int_weights=True, int_scale=65536.0)
+    print("case: paired cube 10k", exec_mode, flush=True)
+    assert check_int(net, 301, kw, exec_mode=exec_mode) > 500
+
+
+@pytest.mark.parametrize("model", [0, 1], ids=["izh", "adex"])
+def test_paired_grid_liquid_with_inputs_bit_exact(model):
+    net, bits = grid_case(model)
+    per_wave = delay1_per_wave(net)
+    print(f"case: paired grid model {model}: delay-1 synapses per wave max {per_wave.max()}",
+          flush=True)
+    assert per_wave.max() > 64          # several pull rounds per wave
+    kw = dict(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=15.0, noise_kick_mv=20.0, noise_sd=0.5,
+              seed=11, int_weights=True, int_scale=65536.0)
+    assert check_int(net, 301, kw, series=bits, exp_mode=1) > 200
+
+
+@pytest.mark.parametrize("scale", [4096.0, 65536.0], ids=["q12", "q16"])
+def test_paired_all_delay1_bit_exact(scale):
+    net = all_delay1_net(3000, 40, seed=5)
+    print("case: paired all delay 1", scale, flush=True)
+    kw = dict(CFG0, int_weights=True, int_scale=scale)
+    assert check_int(net, 257, kw) > 500
+
+
+def test_paired_float_ring_tolerance():
+    """Float ring: v, u, I of every neuron whose spike count equals the
+    oracle's within 1e-4 relative over 200 steps (denominator max(|x|, 1)),
+    at most 0.1 % of neurons with a different count."""
+    net = N.cube_net(10000, mean_deg=100.0, seed=1, w_scale=0.7)
+    kw = dict(CFG0, int_weights=False)
+    sim = Sim(net, SimConfig(step_shape=2, **kw))
+    assert sim.launch_shape() == 2
+    ref = O.OracleSim(net, O.default_cfg(**kw))
+    worst, diverged = 0.0, 0
+    for _ in range(10):
+        sim.run_steps(20)
+        ref.run_steps(20)
+        same = sim.pn_total() == ref.pn_total()
+        diverged = max(diverged, int((~same).sum()))
+        for g, r in ((sim.state(0), ref.state(0)), (sim.state(1), ref.state(1)),
+                     (sim.syn_current(), ref.syn_current())):
+            rel = np.abs(g.astype(np.float64) - r) / np.maximum(np.abs(r), 1.0)
+            worst = max(worst, float(rel[same].max()))
+    print(f"max rel error {worst:.2e}, diverged {diverged}")
+    assert worst < 1e-4
+    assert diverged <= 10
+
+
+def test_paired_falls_back_when_watching():
+    """A watched neuron needs the state after every step: the sim runs one
+    step per launch while watching, paired again after, same raster."""
+    net = N.cube_net(6000, mean_deg=80.0, seed=4, w_scale=0.7)
+    kw = dict(CFG0, int_weights=True)
+    sim = Sim(net, SimConfig(raster_steps=300, step_shape=2, **kw))
+    assert sim.launch_shape() == 2
+    sim.run_steps(100)
+    sim.set_v_watch([3, 77, 5000])
+    assert sim.launch_shape() == 1
+    sim.run_steps(101)
+    sim.set_v_watch([])
+    assert sim.launch_shape() == 2
+    sim.run_steps(99)
+    _, cnt, ids = sim.read_spikes()
+    rc, ri = O.OracleSim(net, O.default_cfg(**kw)).run_steps(300, record=True)
+    assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
+
+
+def test_paired_batch_equals_alone():
+    """Packed networks in paired launches: each network gets the raster of
+    its one-step-per-launch run alone."""
+    from neuro_edit_v2_amd.batch import Batch
+    steps = 201
+    nets, series = [], []
+    for k, n in enumerate([700, 2197, 64, 1331, 3000]):
+        nets.append(N.grid_net(0, model=k % 2, seed=k + 3, dt_ms=0.5, dims=W.grid_dims(n)))
+        series.append(W.stream_window(W.rng_stream(100 + k, 8, 0.05), 8, steps))
+    cfg = [dict(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=15.0 + k, noise_kick_mv=20.0,
+                noise_sd=0.5, seed=11 * k + 1, int_weights=True, raster_steps=steps)
+           for k in range(len(nets))]
+    b = Batch(nets, [SimConfig(step_shape=2, **c) for c in cfg])
+    for k in range(len(nets)):
+        for q in range(8):
+            b.play_input(k, q, series[k][:, q])
+    b.run_steps(steps)
+    for k, net in enumerate(nets):
+        s = Sim(net, SimConfig(step_shape=1, **cfg[k]))
+        for q in range(8):
+            s.play_input(q, series[k][:, q])
+        s.run_steps(steps)
+        _, sc, si = s.read_spikes()
+        s.close()
+        _, bc, bi = b.read_spikes(k)
+        assert np.array_equal(bc, sc) and np.array_equal(bi, si), f"net {k}"
+
+
+def test_autotune_tries_paired():
+    net = N.cube_net(5000, mean_deg=60.0, seed=21)
+    kw = dict(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=30.0, noise_sd=1.0, seed=3,
+              int_weights=True)
+    g = Sim(net, SimConfig(raster_steps=300, **kw))
+    rep = g.autotune(64)
+    print(rep)
+    assert "paired" in rep and "chosen" in rep
+    g.run_steps(300)
+    _, c, ids = g.read_spikes()
+    rc, ri = O.OracleSim(net, O.default_cfg(**kw)).run_steps(300, record=True)
+    assert np.array_equal(c, rc) and np.array_equal(ids, ri)
