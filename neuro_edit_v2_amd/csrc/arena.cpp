@@ -889,9 +889,11 @@ int Arena::run(uint64_t n_steps) {
         return NH_OK;
     }
     if (use_graph) {
-        // graph chunk: a multiple of D near the requested length
+        // graph chunk: a multiple of D near the requested length (and even
+        // in the paired shape, so that every launch of a chunk is a pair)
         const uint32_t want = std::max<uint32_t>(graph_steps_req, 1);
-        const uint32_t g = std::max<uint32_t>(1, (want + D / 2) / D) * D;
+        const uint32_t unit = (D % 2 && pair_usable()) ? 2 * D : D;
+        const uint32_t g = std::max<uint32_t>(1, (want + unit / 2) / unit) * unit;
         if (g != graph_steps) {
             graph_steps = g;
             graph_dirty = true;
