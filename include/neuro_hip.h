@@ -200,6 +200,13 @@ int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_c
 /* Launch shape the next nh_run_steps uses (addition): 0 two kernels per
  * step, 1 one fused kernel per step, 2 two steps per launch. */
 int nh_launch_shape(nh_sim *s, int32_t *shape);
+/* Test hook for the failure path of the two-steps-per-launch shape (no
+ * counterpart in the reference): spin_limit = polls before a waiting wave
+ * gives up (0 = the default), mute_wave0 = 1 makes wave 0 withhold its
+ * published spike word, so the waves that wait for it give up, the launch
+ * ends early and the next synchronising call reports NH_ERR_HIP.  Never set
+ * outside tests; nh_reset clears the failure. */
+int nh_test_pair_wait(nh_sim *s, uint32_t spin_limit, uint32_t mute_wave0);
 
 /* Measurement helpers (additions, no counterpart in the reference).
  * nh_probe_copy: copy `bytes` between two device buffers with a plain

@@ -604,6 +604,8 @@ StepArgs Arena::args() const {
     a.pl_w = d_pl_w.p;
     a.pub = d_pub.as<uint64_t>();
     a.pair_abort = d_pair_abort.as<uint32_t>();
+    a.pair_spins = pair_spins ? pair_spins : kPairSpinsDefault;
+    a.pair_mute = pair_mute;
     a.xrec = d_xrec.as<float2>();
     a.xrec_depth = xrec_depth;
     return a;

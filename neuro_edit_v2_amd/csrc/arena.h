@@ -147,6 +147,9 @@ class Arena {
     bool pair_tables = false;
     bool pair_launched = false;          // paired launches since the last abort check
     uint32_t pair_cap_block = 0, pair_cap_waves = 0;
+    // test hook (failure handling): poll limit of the pair wait (0 = the
+    // default), and whether wave 0 withholds its published word
+    uint32_t pair_spins = 0, pair_mute = 0;
     // true when the next chunk may use the paired kernel
     bool pair_usable();
     int check_pair_abort();

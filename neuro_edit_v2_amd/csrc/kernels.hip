@@ -469,10 +469,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
 //
 // The wait needs every source wave to run while its consumers wait, so the
 // launcher uses this kernel only when the whole grid fits on the device at
-// once; the wait is bounded anyway: past kPairSpins polls a wave raises
+// once; the wait is bounded anyway: past a.pair_spins polls (default
+// kPairSpinsDefault) a wave raises
 // pair_abort (the host reports the launch as failed) and moves on, and
 // every other waiting wave sees the flag and stops waiting.
-constexpr uint32_t kPairSpins = 1u << 16;
 
 template <typename RT>
 __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t ent, RT w,
@@ -484,7 +484,7 @@ __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t en
         if (has) v = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const bool ok = !has || (uint32_t)(v >> 32) == tag;
         if (__ballot(!ok) == 0ull) break;
-        if (spins >= kPairSpins ||
+        if (spins >= a.pair_spins ||
             ((spins & 63u) == 63u &&
              __hip_atomic_load(a.pair_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
             if (lane == 0)
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
                                                 ext, s, xs);
     const uint64_t m0 = __ballot(sp0);
     const uint32_t tag = (uint32_t)t0 + 1u;   // never 0 before 2^32 steps: 0 is "unwritten"
-    if (lane < 2u) {
+    if (lane < 2u && !(a.pair_mute && wave == 0)) {
         const uint32_t half = lane ? (uint32_t)(m0 >> 32) : (uint32_t)m0;
         __hip_atomic_store(&a.pub[2u * wave + lane], ((uint64_t)tag << 32) | half, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);

@@ -15,6 +15,7 @@ constexpr int kBlock = 256;          // default workgroup: 4 waves
 constexpr int kMaxBlock = 512;       // largest workgroup the step kernels accept
 constexpr int kMaxParams = 10;       // AdEx row width
 constexpr uint32_t kNoSpike = 0xFFFFFFFFu;
+constexpr uint32_t kPairSpinsDefault = 1u << 16;  // polls before a paired-step wave gives up
 
 // Per-network scalars (one row per packed network).  All fields are read
 // with scalar loads (the index is wave-uniform).
@@ -133,6 +134,8 @@ struct StepArgs {
     const void *pl_w;
     uint64_t *pub;
     uint32_t *pair_abort;
+    uint32_t pair_spins;         // polls before a wave gives up waiting
+    uint32_t pair_mute;          // test only: 1 = wave 0 never publishes
     // watched neurons (record ring of state rows)
     const uint32_t *watch;
     uint32_t n_watch;

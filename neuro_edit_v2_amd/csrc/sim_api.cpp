@@ -204,6 +204,15 @@ int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *mai
     return NH_OK;
 }
 
+int nh_test_pair_wait(nh_sim *s, uint32_t spin_limit, uint32_t mute_wave0) {
+    CHECK_SIM(s);
+    TRY(s->ar.sync());
+    s->ar.pair_spins = spin_limit;
+    s->ar.pair_mute = mute_wave0 ? 1u : 0u;
+    s->ar.graph_dirty = true;  // both ride in the captured kernel arguments
+    return NH_OK;
+}
+
 int nh_launch_shape(nh_sim *s, int32_t *shape) {
     CHECK_SIM(s);
     if (!shape) return set_error(NH_ERR_ARG, "null output");

@@ -7,12 +7,14 @@ local wiring, stimulus inputs, Izhikevich and AdEx), all of them, with more
 than 64 delay-1 synapses into one wave -- over odd step counts (a one-step
 launch closes each chunk), with graph and direct launches.  Float ring:
 within the tolerances of the fused shape.  Where the shape cannot run
-(watched neurons) the step falls back to one launch per step.
+(watched neurons) the step falls back to one launch per step.  The bounded
+wait's failure path, forced from the host, ends the launch and reports.
 """
 import numpy as np
 import pytest
 
 from neuro_edit_v2_amd import Sim, SimConfig
+from neuro_edit_v2_amd import _lib as L
 from neuro_edit_v2_amd import net as N
 from neuro_edit_v2_amd import workloads as W
 from oracle import oracle as O
@@ -181,3 +183,28 @@ def test_autotune_tries_paired():
     _, c, ids = g.read_spikes()
     rc, ri = O.OracleSim(net, O.default_cfg(**kw)).run_steps(300, record=True)
     assert np.array_equal(c, rc) and np.array_equal(ids, ri)
+
+
+def test_paired_wait_gives_up_and_reports():
+    """Failure path of the pair wait, forced from the host: wave 0 withholds
+    its published word and the poll limit is 64, so the waves it feeds give
+    up; every launch still ends, the next synchronisation reports the
+    failure, and after a reset the sim runs bit-exact again."""
+    net = all_delay1_net(3000, 40, seed=5)
+    src0 = net["tgt"][: 64 * 40]                 # the synapses of wave 0's neurons
+    assert len(np.unique(src0 // 64)) == (3000 + 63) // 64   # wave 0 feeds every wave
+    kw = dict(CFG0, int_weights=True)
+    sim = Sim(net, SimConfig(raster_steps=200, step_shape=2, **kw))
+    assert sim.launch_shape() == 2
+    L.check(L.lib().nh_test_pair_wait(sim._h, 64, 1))
+    print("case: paired wait, wave 0 muted, 64 polls", flush=True)
+    sim.run_steps(20)
+    with pytest.raises(L.NHError) as e:
+        sim.synchronize()
+    assert e.value.code == L.NH_ERR_HIP and "reset" in str(e.value)
+    L.check(L.lib().nh_test_pair_wait(sim._h, 0, 0))
+    sim.reset_state()
+    sim.run_steps(200)
+    _, cnt, ids = sim.read_spikes()
+    rc, ri = O.OracleSim(net, O.default_cfg(**kw)).run_steps(200, record=True)
+    assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
