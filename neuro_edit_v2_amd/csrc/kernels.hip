@@ -236,17 +236,39 @@ __device__ __forceinline__ void push_custom(const StepArgs &a, uint32_t src, uin
     }
 }
 
-// One neuron's update at step t, in the order of sim/src/cpu.rs:217-260:
-// ring arrival rv, custom-tau traces (ring slot `slot`), stimulus (window
-// row kk, skipped when stim_on is false), Poisson kick, Gaussian current,
-// model step.  s holds the synaptic current before the step and returns it
-// decayed; xs is the model state.  Returns whether the neuron spiked.
-template <int RING, int PMODE, int MMODE>
+// Noise of one neuron at local step tl: whether the Poisson kick fires and
+// the Gaussian draw (0 when off).  Depends on the step, the neuron and the
+// seed only, so a kernel may compute it before the step's input is known.
+struct Noise {
+    bool kick;
+    float g;
+};
+__device__ __forceinline__ Noise noise_terms(const NetRec &nr, uint32_t local, uint64_t tl) {
+    Noise z{false, 0.0f};
+    if (nr.pkick > 0.0f || nr.sd != 0.0f) {
+        // one hash serves the kick (high half) and the Gaussian (low half)
+        const uint64_t key = noise_key(local, tl, nr.seed);
+        const uint64_t h0 = mix64(key);
+        z.kick = nr.pkick > 0.0f && kick_fires(h0, nr.pkick);
+        if (nr.sd != 0.0f) z.g = gauss_draw(h0, key);
+    }
+    return z;
+}
+
+// One neuron's update at step t (local step tl), in the order of
+// sim/src/cpu.rs:217-260: ring arrival rv, custom-tau traces (ring slot
+// `slot`), stimulus (window row kk, skipped when stim_on is false), Poisson
+// kick and Gaussian current, model step.  With PRE the noise comes in nz
+// (computed earlier by the caller); otherwise it is computed here, after
+// the parameter rows.  s holds the synaptic current before the step and
+// returns it decayed; xs is the model state.  Returns whether the neuron
+// spiked.
+template <int RING, int PMODE, int MMODE, bool PRE>
 __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &nr, uint32_t i,
                                               uint32_t local, uint64_t tl, uint32_t slot,
                                               uint32_t kk, bool stim_on, uint32_t prow,
-                                              typename Ring<RING>::T rv, float ext, float &s,
-                                              float2 &xs) {
+                                              typename Ring<RING>::T rv, float ext,
+                                              const Noise &nz, float &s, float2 &xs) {
     using RT = typename Ring<RING>::T;
     bool spiked;
     const RingScale rsc(nr.int_scale);
@@ -309,7 +331,10 @@ __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &n
     const bool is_adex = MMODE == 1 || (MMODE == 2 && model == 1);
     // sim/src/cpu.rs:240-245: kick * model gain (C for AdEx, adex.rs:189-191)
     const float gain = is_adex ? pr[0] : 1.0f;
-    if (nr.pkick > 0.0f || nr.sd != 0.0f) {
+    if (PRE) {
+        if (nz.kick) I += nr.kick * gain;
+        if (nr.sd != 0.0f) I += (nr.sd * nz.g) * gain;
+    } else if (nr.pkick > 0.0f || nr.sd != 0.0f) {
         // one hash serves the kick (high half) and the Gaussian (low half)
         const uint64_t key = noise_key(local, tl, nr.seed);
         const uint64_t h0 = mix64(key);
@@ -405,8 +430,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     bool spiked = false;
     float s_new = s_old;
     if (act)
-        spiked = neuron_update<RING, PMODE, MMODE>(a, nr, i, local, tl, slot, a.k, stim_on, prow, rv,
-                                                   ext, s_new, xs);
+        spiked = neuron_update<RING, PMODE, MMODE, false>(a, nr, i, local, tl, slot, a.k, stim_on,
+                                                          prow, rv, ext, Noise{false, 0.0f}, s_new,
+                                                          xs);
 
     // ---- wave64 ballot: the mask IS this wave's word of the spike bitmap
     const uint64_t mask = __ballot(spiked);
@@ -547,8 +573,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     const bool act0 = local < nr.n && nr.active && (nr.t_len == 0 || tl0 < nr.t_len);
     bool sp0 = false;
     if (act0)
-        sp0 = neuron_update<RING, PMODE, MMODE>(a, nr, i, local, tl0, slot0, a.k, stim0, prow, rv0,
-                                                ext, s, xs);
+        sp0 = neuron_update<RING, PMODE, MMODE, false>(a, nr, i, local, tl0, slot0, a.k, stim0, prow,
+                                                       rv0, ext, Noise{false, 0.0f}, s, xs);
     const uint64_t m0 = __ballot(sp0);
     const uint32_t tag = (uint32_t)t0 + 1u;   // never 0 before 2^32 steps: 0 is "unwritten"
     if (lane < 2u && !(a.pair_mute && wave == 0)) {
@@ -563,6 +589,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     if (m0) push_wave<RING, PACKED, true>(a, m0, row_lo, row_hi, slot0, lane);
     if (sp0) spike_marks(a, nr, i, t0, tl0);
     if (lane == 0) spike_words(a, wave, t0, rast_origin, m0);
+
+    // step t1's noise does not depend on the pulled input: computed before
+    // the wait, it leaves the wait's end with less work behind it
+    const uint64_t t1 = t0 + 1u, tl1 = tl0 + 1u;
+    const bool act1 = local < nr.n && nr.active && (nr.t_len == 0 || tl1 < nr.t_len);
+    const Noise nz1 = act1 ? noise_terms(nr, local, tl1) : Noise{false, 0.0f};
 
     // ---- pull the delay-1 arrivals of t0 into this wave's neurons
     RT *acc_w = acc + (threadIdx.x & ~(uint32_t)(kWave - 1));
@@ -582,12 +614,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     const RT rv1p = rv1 + acc_w[lane];
 
     // ---- step t1
-    const uint64_t t1 = t0 + 1u, tl1 = tl0 + 1u;
-    const bool act1 = local < nr.n && nr.active && (nr.t_len == 0 || tl1 < nr.t_len);
     bool sp1 = false;
     if (act1)
-        sp1 = neuron_update<RING, PMODE, MMODE>(a, nr, i, local, tl1, slot1, a.k + 1u, stim1, prow,
-                                                rv1p, ext, s, xs);
+        sp1 = neuron_update<RING, PMODE, MMODE, true>(a, nr, i, local, tl1, slot1, a.k + 1u, stim1,
+                                                      prow, rv1p, ext, nz1, s, xs);
     const uint64_t m1 = __ballot(sp1);
     if (m1) push_wave<RING, PACKED, false>(a, m1, row_lo, row_hi, slot1, lane);
 
