@@ -156,6 +156,44 @@ template <class T> int put(nh::DBuf &b, uint64_t off, const T *src, uint64_t cou
 
 void update_grid(nh_server *s);
 
+// Rows [i0, i1) of a request's CSR into its device layout: each row sorted
+// by (delay, target), ties in input order, the local target packed with the
+// delay (key = packed word << 32 | input index, so integer order is that
+// order), weights quantised for the int ring.  Every synapse is kept, so
+// row i lands at src_ptr[i] and ranges of rows are independent.  Returns
+// an error message or null.
+const char *prep_rows(const nh_net_desc *d, const nh_config *cfg, uint32_t max_delay, bool int_w,
+                      uint32_t i0, uint32_t i1, Request *r) {
+    std::vector<uint64_t> key;
+    for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t lo = d->src_ptr[i], hi = d->src_ptr[i + 1], m = hi - lo;
+        key.resize(m);
+        for (uint32_t j = lo; j < hi; ++j) {
+            if (d->tgt[j] >= d->n) return "synapse target out of range";
+            if (d->delay[j] < 1 || d->delay[j] > max_delay) return "delay outside 1..max_delay of the server";
+            if (d->tau && d->tau[j] > 0.0f) return "custom-tau synapses are not supported";
+            key[j - lo] = ((uint64_t)(d->tgt[j] | ((uint32_t)d->delay[j] << 27)) << 32) | j;
+        }
+        if (m > 32) {
+            std::sort(key.begin(), key.end());
+        } else {
+            for (uint32_t a = 1; a < m; ++a) {  // short rows: insertion sort
+                const uint64_t v = key[a];
+                uint32_t b = a;
+                for (; b && key[b - 1] > v; --b) key[b] = key[b - 1];
+                key[b] = v;
+            }
+        }
+        for (uint32_t q = 0; q < m; ++q) {
+            const uint32_t j = (uint32_t)key[q];
+            r->syn[lo + q] = (uint32_t)(key[q] >> 32);
+            if (int_w) r->wq[lo + q] = (int32_t)std::lrintf(d->w[j] * cfg->int_scale);
+            else r->wf[lo + q] = d->w[j];
+        }
+    }
+    return nullptr;
+}
+
 // Stimulus window of every slot: local step (t - t0) of each active request.
 void pack_server(void *ctx, uint64_t t0, uint32_t k, float *out) {
     nh_server *s = (nh_server *)ctx;
@@ -549,35 +587,29 @@ int nh_server_submit(nh_server *s, const nh_net_desc *d, const nh_config *cfg, c
     if (r->n_al / 64 > s->cfg.cap_neurons / 64 || r->E > s->cfg.cap_synapses)
         return set_error(NH_ERR_CAPACITY, "request larger than the server arena");
     // rows sorted by (delay, target), local targets packed with the delay
-    r->row_off.assign(r->n + 1, 0);
+    // (src_ptr is checked monotone from 0 by nh_check_net); large requests
+    // split their rows over a few threads: the preparation is part of the
+    // request's latency
+    if (r->n) r->row_off.assign(d->src_ptr, d->src_ptr + r->n + 1);
+    else r->row_off.assign(1, 0);
     r->syn.resize(r->E);
     if (s->cfg.int_weights) r->wq.resize(r->E);
     else r->wf.resize(r->E);
-    std::vector<uint32_t> idx;
-    uint64_t o = 0;
-    for (uint32_t i = 0; i < r->n; ++i) {
-        const uint32_t lo = d->src_ptr[i], hi = d->src_ptr[i + 1];
-        if (hi < lo) return set_error(NH_ERR_ARG, "src_ptr not monotone");
-        r->row_off[i] = (uint32_t)o;
-        idx.resize(hi - lo);
-        std::iota(idx.begin(), idx.end(), lo);
-        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
-            if (d->delay[x] != d->delay[y]) return d->delay[x] < d->delay[y];
-            return d->tgt[x] < d->tgt[y];
-        });
-        for (uint32_t j : idx) {
-            if (d->tgt[j] >= d->n) return set_error(NH_ERR_ARG, "synapse target out of range");
-            if (d->delay[j] < 1 || d->delay[j] > s->cfg.max_delay)
-                return set_error(NH_ERR_ARG, "delay outside 1..max_delay of the server");
-            if (d->tau && d->tau[j] > 0.0f)
-                return set_error(NH_ERR_UNSUPPORTED, "custom-tau synapses are not supported");
-            r->syn[o] = d->tgt[j] | ((uint32_t)d->delay[j] << 27);
-            if (s->cfg.int_weights) r->wq[o] = (int32_t)std::lrintf(d->w[j] * cfg->int_scale);
-            else r->wf[o] = d->w[j];
-            ++o;
-        }
+    {
+        const bool iw = s->cfg.int_weights != 0;
+        const uint32_t nt = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(8, r->E / 65536 + 1), r->n ? r->n : 1);
+        std::vector<const char *> err(nt, nullptr);
+        std::vector<std::thread> pool;
+        for (uint32_t q = 1; q < nt; ++q)
+            pool.emplace_back([&, q] {
+                err[q] = prep_rows(d, cfg, s->cfg.max_delay, iw, (uint32_t)((uint64_t)r->n * q / nt),
+                                   (uint32_t)((uint64_t)r->n * (q + 1) / nt), r.get());
+            });
+        err[0] = prep_rows(d, cfg, s->cfg.max_delay, iw, 0, (uint32_t)(r->n / nt), r.get());
+        for (auto &th : pool) th.join();
+        for (const char *e : err)
+            if (e) return set_error(std::strstr(e, "custom-tau") ? NH_ERR_UNSUPPORTED : NH_ERR_ARG, e);
     }
-    r->row_off[r->n] = (uint32_t)o;
     // distinct parameter rows
     r->row_of.resize(r->n);
     for (uint32_t i = 0; i < r->n; ++i) {
