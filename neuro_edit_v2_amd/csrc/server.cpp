@@ -11,6 +11,7 @@
 // sees only its own local step (noise hash, stimulus, windows), so its
 // result equals the plain path's result for it alone.
 #include <algorithm>
+#include <cassert>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -112,6 +113,38 @@ struct Request {
     std::vector<uint32_t> counts;     // [n_windows][n]
 };
 
+// Pinned host staging for admission uploads: every array of a request is
+// packed here and copied with asynchronous copies (pageable sources would
+// make each copy a synchronous staged one).  Reused; grown on demand; free
+// again once the admission's stream synchronisation returns.
+struct Stage {
+    uint8_t *buf = nullptr;
+    size_t cap = 0, used = 0;
+    Stage() = default;
+    Stage(const Stage &) = delete;
+    Stage &operator=(const Stage &) = delete;
+    ~Stage() {
+        if (buf) (void)hipHostFree(buf);
+    }
+    hipError_t reserve(size_t bytes) {
+        used = 0;
+        if (bytes <= cap) return hipSuccess;
+        if (buf) (void)hipHostFree(buf);
+        buf = nullptr;
+        cap = 0;
+        const size_t want = bytes + bytes / 2;
+        hipError_t e = hipHostMalloc((void **)&buf, want, 0);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    template <class T> T *take(uint64_t count) {
+        T *p = (T *)(buf + used);
+        used += (count * sizeof(T) + 255) & ~size_t(255);
+        assert(used <= cap);
+        return p;
+    }
+};
+
 }  // namespace
 
 struct nh_server {
@@ -126,6 +159,7 @@ struct nh_server {
     std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
     std::vector<Request *> slot_req;
     FreeList fn, fs, fst;             // neurons (64-units), synapses, stimulus entries
+    Stage stage;                      // pinned staging of admission uploads
     std::vector<nh::PRow> table;
     nh_server_stats st{};
     uint64_t active = 0;
@@ -287,6 +321,22 @@ int build_arena(nh_server *s) {
     s->st.cap_synapses = c.cap_synapses;
     update_grid(s);
     TRY(ar.run(0));  // fixes the round length (a multiple of D) and captures the graph
+    // first-use costs off the first request's latency: the pinned staging
+    // (sized for requests up to the capacities, at most 64 MiB; it grows if
+    // a larger request comes) and the first launch of the range-clear kernel
+    const uint64_t stage = std::min<uint64_t>(64ull << 20, c.cap_synapses * 8 + ar.n_total * 25 + (1u << 20));
+    HIPCHK(s->stage.reserve(stage));
+    nh::launch_clear_range(ar.args(), 0, 64, ar.stream);
+    // and the first pinned host-to-device copies, large and small (into the
+    // unused synapse area and the inactive slot records)
+    const uint64_t big = std::min<uint64_t>(1u << 20, c.cap_synapses);
+    uint32_t *z = s->stage.take<uint32_t>(big);
+    std::memset(z, 0, big * 4);
+    TRY(put(ar.d_syn, 0, z, big, ar.stream));
+    nh::NetRec *r0 = s->stage.take<nh::NetRec>(1);
+    *r0 = ar.recs[0];
+    TRY(put(ar.d_nets, 0, r0, 1, ar.stream));
+    HIPCHK(hipStreamSynchronize(ar.stream));
     return NH_OK;
 }
 
@@ -294,6 +344,15 @@ int build_arena(nh_server *s) {
 int admit(nh_server *s, Request *r) {
     Arena &ar = s->ar;
     hipStream_t st = ar.stream;
+    // everything below is packed into the pinned stage (256-byte aligned
+    // pieces) and copied asynchronously; the stream is synchronised at the end
+    const uint64_t na = r->n_al;
+    // (each piece rounds up to 256 B: 16 pieces of slack cover the fixed
+    // ones, every new parameter row is a piece of its own)
+    const size_t need = 256 * 16 + r->rows.size() * 256 + na * (1 + 8 + 8 + 8) + (size_t)r->E * 8 +
+                        (size_t)r->n_stim * 8 + na / 64 * 2 + sizeof(nh::NetRec);
+    HIPCHK(s->stage.reserve(need));
+    Stage &sg = s->stage;
     // parameter rows into the shared table
     std::vector<uint8_t> map_rows(r->rows.size());
     for (size_t q = 0; q < r->rows.size(); ++q) {
@@ -303,49 +362,61 @@ int admit(nh_server *s, Request *r) {
         if (k == s->table.size()) {
             if (k >= 256) return set_error(NH_ERR_CAPACITY, "more than 256 distinct parameter rows");
             s->table.push_back(r->rows[q]);
-            TRY(put(ar.d_ptab, k, &s->table[k], 1, st));
+            nh::PRow *pr = sg.take<nh::PRow>(1);
+            *pr = r->rows[q];
+            TRY(put(ar.d_ptab, k, pr, 1, st));
         }
         map_rows[q] = (uint8_t)k;
     }
     const uint64_t b = r->base;
-    std::vector<uint8_t> prow(r->n_al, 0);
+    uint8_t *prow = sg.take<uint8_t>(na);
+    std::memset(prow, 0, na);
     for (uint32_t i = 0; i < r->n; ++i) prow[i] = map_rows[r->row_of[i]];
-    TRY(put(ar.d_prow, b, prow.data(), r->n_al, st));
-    std::vector<uint32_t> lo(r->n_al), hi(r->n_al);
+    TRY(put(ar.d_prow, b, prow, na, st));
+    uint32_t *lo = sg.take<uint32_t>(na), *hi = sg.take<uint32_t>(na);
     for (uint32_t i = 0; i < r->n_al; ++i) {
         const uint32_t a = i < r->n ? r->row_off[i] : r->row_off[r->n];
         const uint32_t z = i < r->n ? r->row_off[i + 1] : r->row_off[r->n];
         lo[i] = (uint32_t)(r->sbase + a);
         hi[i] = (uint32_t)(r->sbase + z);
     }
-    TRY(put(ar.d_src_ptr, b, lo.data(), r->n_al, st));
-    TRY(put(ar.d_row_hi, b, hi.data(), r->n_al, st));
-    std::vector<uint32_t> syn(r->E);
+    TRY(put(ar.d_src_ptr, b, lo, na, st));
+    TRY(put(ar.d_row_hi, b, hi, na, st));
+    uint32_t *syn = sg.take<uint32_t>(r->E);
     for (uint64_t j = 0; j < r->E; ++j) {
         const uint32_t tg = (r->syn[j] & 0x07FFFFFFu) + (uint32_t)b;
         syn[j] = tg | (r->syn[j] & 0xF8000000u);
     }
-    TRY(put(ar.d_syn, r->sbase, syn.data(), r->E, st));
-    if (ar.ring_int) TRY(put(ar.d_w, r->sbase, r->wq.data(), r->E, st));
-    else TRY(put(ar.d_w, r->sbase, r->wf.data(), r->E, st));
+    TRY(put(ar.d_syn, r->sbase, syn, r->E, st));
+    uint32_t *wst = sg.take<uint32_t>(r->E);
+    if (r->E) std::memcpy(wst, ar.ring_int ? (const void *)r->wq.data() : (const void *)r->wf.data(), r->E * 4);
+    TRY(put(ar.d_w, r->sbase, wst, r->E, st));
     if (s->cfg.n_inputs) {
-        std::vector<uint32_t> slo(r->n_al), shi(r->n_al);
+        uint32_t *slo = sg.take<uint32_t>(na), *shi = sg.take<uint32_t>(na);
         for (uint32_t i = 0; i < r->n_al; ++i) {
             const uint32_t a = i < r->n ? r->stim_off[i] : r->stim_off[r->n];
             const uint32_t z = i < r->n ? r->stim_off[i + 1] : r->stim_off[r->n];
             slo[i] = (uint32_t)(r->stbase + a);
             shi[i] = (uint32_t)(r->stbase + z);
         }
-        TRY(put(ar.d_stim_ptr, b, slo.data(), r->n_al, st));
-        TRY(put(ar.d_stim_hi, b, shi.data(), r->n_al, st));
-        TRY(put(ar.d_stim_input, r->stbase, r->stim_in.data(), r->n_stim, st));
-        TRY(put(ar.d_stim_w, r->stbase, r->stim_w.data(), r->n_stim, st));
+        TRY(put(ar.d_stim_ptr, b, slo, na, st));
+        TRY(put(ar.d_stim_hi, b, shi, na, st));
+        uint32_t *sin = sg.take<uint32_t>(r->n_stim);
+        float *sw = sg.take<float>(r->n_stim);
+        if (r->n_stim) {
+            std::memcpy(sin, r->stim_in.data(), r->n_stim * 4);
+            std::memcpy(sw, r->stim_w.data(), r->n_stim * 4);
+        }
+        TRY(put(ar.d_stim_input, r->stbase, sin, r->n_stim, st));
+        TRY(put(ar.d_stim_w, r->stbase, sw, r->n_stim, st));
     }
-    std::vector<float2> x0(r->n_al, make_float2(0.f, 0.f));
-    std::copy(r->x0.begin(), r->x0.end(), x0.begin());
-    TRY(put(ar.d_x, b, x0.data(), r->n_al, st));
-    std::vector<uint16_t> wn(r->n_al / 64, (uint16_t)r->slot);
-    TRY(put(ar.d_wave_net, b / 64, wn.data(), wn.size(), st));
+    float2 *x0 = sg.take<float2>(na);
+    std::fill(x0, x0 + na, make_float2(0.f, 0.f));
+    std::copy(r->x0.begin(), r->x0.end(), x0);
+    TRY(put(ar.d_x, b, x0, na, st));
+    uint16_t *wn = sg.take<uint16_t>(na / 64);
+    std::fill(wn, wn + na / 64, (uint16_t)r->slot);
+    TRY(put(ar.d_wave_net, b / 64, wn, na / 64, st));
     // ring columns, s, counters of the range start at zero
     nh::StepArgs a = ar.args();
     nh::launch_clear_range(a, (uint32_t)b, r->n_al, st);
@@ -364,7 +435,9 @@ int admit(nh_server *s, Request *r) {
     rec.win_steps = r->win_steps;
     // the request stops exactly after its own steps, whatever the round length
     rec.t_len = (uint32_t)r->total_steps;
-    TRY(put(ar.d_nets, r->slot, &rec, 1, st));
+    nh::NetRec *rs = sg.take<nh::NetRec>(1);
+    *rs = rec;
+    TRY(put(ar.d_nets, r->slot, rs, 1, st));
     HIPCHK(hipStreamSynchronize(st));
     r->t0 = ar.t;
     r->admit_ns = now_ns();
