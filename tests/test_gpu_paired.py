@@ -208,3 +208,20 @@ def test_paired_wait_gives_up_and_reports():
     _, cnt, ids = sim.read_spikes()
     rc, ri = O.OracleSim(net, O.default_cfg(**kw)).run_steps(200, record=True)
     assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
+
+
+@pytest.mark.parametrize("window", [37, 64], ids=["odd_window", "even_window"])
+def test_paired_with_raster_drains_mid_run(window):
+    """A raster window smaller than the run makes the host drain it mid-run,
+    which cuts chunks at arbitrary (also odd) lengths: pairs and one-step
+    launches interleave, and the raster is still the oracle's."""
+    net = N.cube_net(8000, mean_deg=80.0, seed=9, w_scale=0.7)
+    kw = dict(CFG0, int_weights=True)
+    sim = Sim(net, SimConfig(raster_steps=window, step_shape=2, **kw))
+    assert sim.launch_shape() == 2
+    for n in (101, 64, 136):
+        sim.run_steps(n)
+    _, cnt, ids = sim.read_spikes()
+    rc, ri = O.OracleSim(net, O.default_cfg(**kw)).run_steps(301, record=True)
+    assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
+    assert rc.sum() > 300
