@@ -7,10 +7,12 @@
 #include "arena.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <thread>
 
 namespace nh {
 
@@ -200,40 +202,73 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     if (ring_int) wq.resize(std::max<uint64_t>(total_syn, 1));
     else wf.resize(std::max<uint64_t>(total_syn, 1));
     {
+        // Row starts first (every synapse without its own tau, in row
+        // order), then the rows themselves, each sorted by (delay, target)
+        // with ties in input order and written at its start.  Rows are
+        // independent, so chunks of rows go to a few threads.
         uint64_t o = 0;
-        std::vector<uint32_t> idx;
         for (uint32_t k = 0; k < nn; ++k) {
             const nh_net_desc &d = nets[k];
             const uint32_t b = slots[k].base;
-            const float scale = cfgs[k].int_scale;
-            for (uint32_t i = 0; i < d.n; ++i) {
-                const uint32_t lo = d.src_ptr[i], hi = d.src_ptr[i + 1];
-                src_ptr[b + i] = (uint32_t)o;
-                idx.resize(hi - lo);
-                std::iota(idx.begin(), idx.end(), lo);
-                std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
-                    if (d.delay[x] != d.delay[y]) return d.delay[x] < d.delay[y];
-                    return d.tgt[x] < d.tgt[y];
-                });
-                for (uint32_t j : idx) {
-                    if (d.tau && d.tau[j] > 0.0f) continue;  // custom-tau: own rows below
-                    const uint32_t tg = b + d.tgt[j];
-                    if (packed) syn[o] = tg | ((uint32_t)d.delay[j] << 27);
-                    else {
-                        syn[o] = tg;
-                        d16[o] = d.delay[j];
+            for (uint32_t i = 0; i < align64(d.n); ++i) {  // padding neurons: empty rows
+                src_ptr[b + i] = (uint32_t)o;               // (the total is checked below)
+                if (i >= d.n) continue;
+                uint64_t c = d.src_ptr[i + 1] - d.src_ptr[i];
+                if (d.tau)
+                    for (uint32_t j = d.src_ptr[i]; j < d.src_ptr[i + 1]; ++j) c -= d.tau[j] > 0.0f ? 1 : 0;
+                o += c;
+            }
+        }
+        if (o > 0xFFFFFFFFull) return set_error(NH_ERR_CAPACITY, "more than 2^32 synapses");
+        src_ptr[n_total] = (uint32_t)o;
+        n_syn = o;
+        struct Rows {
+            uint32_t k, i0, i1;
+        };
+        std::vector<Rows> tasks;
+        for (uint32_t k = 0; k < nn; ++k)
+            for (uint32_t i = 0; i < nets[k].n; i += 2048)
+                tasks.push_back({k, i, std::min<uint32_t>(nets[k].n, i + 2048)});
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            // key: (delay << 32 | target, input index) -- a total order, the
+            // stable (delay, target) order
+            std::vector<std::pair<uint64_t, uint32_t>> key;
+            for (size_t q; (q = next.fetch_add(1)) < tasks.size();) {
+                const nh_net_desc &d = nets[tasks[q].k];
+                const uint32_t b = slots[tasks[q].k].base;
+                const float scale = cfgs[tasks[q].k].int_scale;
+                for (uint32_t i = tasks[q].i0; i < tasks[q].i1; ++i) {
+                    key.clear();
+                    for (uint32_t j = d.src_ptr[i]; j < d.src_ptr[i + 1]; ++j)
+                        if (!(d.tau && d.tau[j] > 0.0f))  // custom-tau: own rows below
+                            key.emplace_back(((uint64_t)d.delay[j] << 32) | d.tgt[j], j);
+                    if (key.size() > 32) {
+                        std::sort(key.begin(), key.end());
+                    } else {
+                        for (size_t a = 1; a < key.size(); ++a) {  // short rows: insertion sort
+                            const auto v = key[a];
+                            size_t c = a;
+                            for (; c && v < key[c - 1]; --c) key[c] = key[c - 1];
+                            key[c] = v;
+                        }
                     }
-                    if (ring_int) wq[o] = quantize(d.w[j], scale);
-                    else wf[o] = d.w[j];
-                    ++o;
+                    uint64_t w = src_ptr[b + i];
+                    for (const auto &kv : key) {
+                        const uint32_t j = kv.second, tg = b + d.tgt[j];
+                        if (packed) syn[w] = tg | ((uint32_t)d.delay[j] << 27);
+                        else {
+                            syn[w] = tg;
+                            d16[w] = d.delay[j];
+                        }
+                        if (ring_int) wq[w] = quantize(d.w[j], scale);
+                        else wf[w] = d.w[j];
+                        ++w;
+                    }
                 }
             }
-            // padding neurons of this network have empty rows
-            for (uint32_t i = d.n; i < align64(d.n); ++i) src_ptr[b + i] = (uint32_t)o;
-        }
-        src_ptr[n_total] = (uint32_t)o;
-        if (o > 0xFFFFFFFFull) return set_error(NH_ERR_CAPACITY, "more than 2^32 synapses");
-        n_syn = o;
+        };
+        run_workers(work, tasks.size());
     }
     // ---- custom-tau synapses (XSyn::build, sim/src/state.rs:245-274): slot k
     // in CSR order per network; per-source rows (k, delay, w) for the push,

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/neuro_hip.h"
+#include "host_util.h"
 #include "layout.h"
 
 namespace nh {

@@ -86,41 +86,45 @@ static int split_raster(nh_batch *b) {
 extern "C" {
 
 int nh_batch_create(uint32_t n_nets, const nh_net_desc *nets, const nh_config *cfgs, nh_batch **out) {
-    if (!nets || !cfgs || !out) return set_error(NH_ERR_ARG, "null argument");
-    *out = nullptr;
-    for (uint32_t k = 0; k < n_nets; ++k) TRY(nh_check_net(&nets[k], &cfgs[k]));
-    int dev = 0;
-    if (hipGetDeviceCount(&dev) != hipSuccess || dev < 1) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
-    std::unique_ptr<nh_batch> b(new (std::nothrow) nh_batch());
-    if (!b) return set_error(NH_ERR_CAPACITY, "out of host memory");
-    for (uint32_t k = 1; k < n_nets; ++k)
-        if (cfgs[k].dt_ms != cfgs[0].dt_ms) return set_error(NH_ERR_ARG, "packed networks must share dt");
-    nh::Tuning tune;
-    tune.fused = cfgs[0].step_shape;
-    tune.use_graph = cfgs[0].exec_mode == 0 ? -1 : (cfgs[0].exec_mode == 2 ? 1 : 0);
-    tune.graph_steps = cfgs[0].graph_steps ? cfgs[0].graph_steps : 64;
-    tune.block = cfgs[0].block;
-    TRY(b->ar.build(n_nets, nets, cfgs, cfgs[0].raster_steps, tune));
-    b->cfgs.assign(cfgs, cfgs + n_nets);
-    b->series.resize(n_nets);
-    b->play_start.resize(n_nets);
-    for (uint32_t k = 0; k < n_nets; ++k) {
-        b->series[k].assign(nets[k].n_inputs, {});
-        b->play_start[k].assign(nets[k].n_inputs, 0);
-    }
-    b->r_cnt.resize(n_nets);
-    b->r_ids.resize(n_nets);
-    b->r_step0.assign(n_nets, 0);
-    b->ar.set_stim_packer(pack_batch, b.get());
-    *out = b.release();
-    return NH_OK;
+    return nh::guarded([&]() -> int {
+        if (!nets || !cfgs || !out) return set_error(NH_ERR_ARG, "null argument");
+        *out = nullptr;
+        for (uint32_t k = 0; k < n_nets; ++k) TRY(nh_check_net(&nets[k], &cfgs[k]));
+        int dev = 0;
+        if (hipGetDeviceCount(&dev) != hipSuccess || dev < 1) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
+        std::unique_ptr<nh_batch> b(new (std::nothrow) nh_batch());
+        if (!b) return set_error(NH_ERR_CAPACITY, "out of host memory");
+        for (uint32_t k = 1; k < n_nets; ++k)
+            if (cfgs[k].dt_ms != cfgs[0].dt_ms) return set_error(NH_ERR_ARG, "packed networks must share dt");
+        nh::Tuning tune;
+        tune.fused = cfgs[0].step_shape;
+        tune.use_graph = cfgs[0].exec_mode == 0 ? -1 : (cfgs[0].exec_mode == 2 ? 1 : 0);
+        tune.graph_steps = cfgs[0].graph_steps ? cfgs[0].graph_steps : 64;
+        tune.block = cfgs[0].block;
+        TRY(b->ar.build(n_nets, nets, cfgs, cfgs[0].raster_steps, tune));
+        b->cfgs.assign(cfgs, cfgs + n_nets);
+        b->series.resize(n_nets);
+        b->play_start.resize(n_nets);
+        for (uint32_t k = 0; k < n_nets; ++k) {
+            b->series[k].assign(nets[k].n_inputs, {});
+            b->play_start[k].assign(nets[k].n_inputs, 0);
+        }
+        b->r_cnt.resize(n_nets);
+        b->r_ids.resize(n_nets);
+        b->r_step0.assign(n_nets, 0);
+        b->ar.set_stim_packer(pack_batch, b.get());
+        *out = b.release();
+        return NH_OK;
+    });
 }
 
 void nh_batch_destroy(nh_batch *b) { delete b; }
 
 int nh_batch_run_steps(nh_batch *b, uint64_t n_steps) {
-    CHECK_B(b);
-    return b->ar.run(n_steps);
+    return nh::guarded([&]() -> int {
+        CHECK_B(b);
+        return b->ar.run(n_steps);
+    });
 }
 
 int nh_batch_sync(nh_batch *b) {
@@ -173,23 +177,25 @@ int nh_batch_read_state(nh_batch *b, uint32_t net, uint32_t slot, float *out, ui
 
 int nh_batch_read_spikes(nh_batch *b, uint32_t net, uint64_t *step0, uint32_t *counts, uint64_t max_steps,
                          uint32_t *ids, uint64_t max_ids, uint64_t *n_steps, uint64_t *n_ids) {
-    CHECK_B(b);
-    if (net >= b->ar.n_nets) return set_error(NH_ERR_ARG, "net out of range");
-    if (!b->ar.rast_steps) return set_error(NH_ERR_UNSUPPORTED, "raster stream off (raster_steps = 0)");
-    TRY(split_raster(b));
-    auto &c = b->r_cnt[net];
-    auto &v = b->r_ids[net];
-    *n_steps = c.size();
-    *n_ids = v.size();
-    *step0 = b->r_step0[net];
-    if (c.size() > max_steps || v.size() > max_ids)
-        return set_error(NH_ERR_CAPACITY, "raster output arrays too small");
-    std::copy(c.begin(), c.end(), counts);
-    std::copy(v.begin(), v.end(), ids);
-    b->r_step0[net] += c.size();
-    c.clear();
-    v.clear();
-    return NH_OK;
+    return nh::guarded([&]() -> int {
+        CHECK_B(b);
+        if (net >= b->ar.n_nets) return set_error(NH_ERR_ARG, "net out of range");
+        if (!b->ar.rast_steps) return set_error(NH_ERR_UNSUPPORTED, "raster stream off (raster_steps = 0)");
+        TRY(split_raster(b));
+        auto &c = b->r_cnt[net];
+        auto &v = b->r_ids[net];
+        *n_steps = c.size();
+        *n_ids = v.size();
+        *step0 = b->r_step0[net];
+        if (c.size() > max_steps || v.size() > max_ids)
+            return set_error(NH_ERR_CAPACITY, "raster output arrays too small");
+        std::copy(c.begin(), c.end(), counts);
+        std::copy(v.begin(), v.end(), ids);
+        b->r_step0[net] += c.size();
+        c.clear();
+        v.clear();
+        return NH_OK;
+    });
 }
 
 int nh_batch_play_input(nh_batch *b, uint32_t net, uint32_t input, const float *series, uint64_t len) {

@@ -21,11 +21,9 @@
 #include <vector>
 
 #include "../../include/neuro_hip.h"
+#include "host_util.h"
 #include "netgen.h"
 
-namespace nh {
-int set_error(int code, const std::string &msg);
-}
 using nh::set_error;
 
 namespace {
@@ -556,36 +554,38 @@ int to_sim(const Net &n, float dt, nh_net &out) {
 extern "C" {
 
 int nh_net_load_ne2(const uint8_t *bytes, uint64_t len, float dt_ms, nh_net **out, nh_ne2_info *info) {
-    if (!bytes || !out) return set_error(NH_ERR_ARG, "null argument");
-    *out = nullptr;
-    Net n;
-    uint32_t version = 0;
-    int rc = decode(bytes, len, n, version);
-    if (rc) return rc;
-    if ((rc = validate(n))) return rc;
-    const float dt = dt_ms > 0.0f ? dt_ms : n.meta.dt_ms;
-    std::unique_ptr<nh_net> net(new (std::nothrow) nh_net());
-    if (!net) return set_error(NH_ERR_CAPACITY, "out of host memory");
-    if ((rc = to_sim(n, dt, *net))) return rc;
-    if (info) {
-        std::memset(info, 0, sizeof(*info));
-        info->version = version;
-        info->dt_ms = n.meta.dt_ms;
-        info->seed = n.meta.seed;
-        info->conduction_speed = n.meta.conduction_speed;
-        info->synapse_tau_ms = n.meta.synapse_tau_ms;
-        info->readout_tau_ms = n.meta.readout_tau_ms;
-        info->noise_rate_hz = n.meta.noise_rate_hz;
-        info->noise_kick_mv = n.meta.noise_kick_mv;
-        info->n_types = (uint32_t)n.types.size();
-        info->n_groups = (uint32_t)n.groups.size();
-        info->n_inputs = (uint32_t)n.inputs.size();
-        info->n_samples = (uint32_t)n.n_samples;
-        info->n_readouts = (uint32_t)n.readouts.size();
-        info->n_scripts = (uint32_t)n.n_scripts;
-    }
-    *out = net.release();
-    return NH_OK;
+    return nh::guarded([&]() -> int {
+        if (!bytes || !out) return set_error(NH_ERR_ARG, "null argument");
+        *out = nullptr;
+        Net n;
+        uint32_t version = 0;
+        int rc = decode(bytes, len, n, version);
+        if (rc) return rc;
+        if ((rc = validate(n))) return rc;
+        const float dt = dt_ms > 0.0f ? dt_ms : n.meta.dt_ms;
+        std::unique_ptr<nh_net> net(new (std::nothrow) nh_net());
+        if (!net) return set_error(NH_ERR_CAPACITY, "out of host memory");
+        if ((rc = to_sim(n, dt, *net))) return rc;
+        if (info) {
+            std::memset(info, 0, sizeof(*info));
+            info->version = version;
+            info->dt_ms = n.meta.dt_ms;
+            info->seed = n.meta.seed;
+            info->conduction_speed = n.meta.conduction_speed;
+            info->synapse_tau_ms = n.meta.synapse_tau_ms;
+            info->readout_tau_ms = n.meta.readout_tau_ms;
+            info->noise_rate_hz = n.meta.noise_rate_hz;
+            info->noise_kick_mv = n.meta.noise_kick_mv;
+            info->n_types = (uint32_t)n.types.size();
+            info->n_groups = (uint32_t)n.groups.size();
+            info->n_inputs = (uint32_t)n.inputs.size();
+            info->n_samples = (uint32_t)n.n_samples;
+            info->n_readouts = (uint32_t)n.readouts.size();
+            info->n_scripts = (uint32_t)n.n_scripts;
+        }
+        *out = net.release();
+        return NH_OK;
+    });
 }
 
 int nh_net_type_name(const nh_net *net, uint32_t k, char *buf, uint32_t cap) {

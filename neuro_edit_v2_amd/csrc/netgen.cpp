@@ -12,11 +12,9 @@
 #include <vector>
 
 #include "../../include/neuro_hip.h"
+#include "host_util.h"
 #include "netgen.h"
 
-namespace nh {
-int set_error(int code, const std::string &msg);
-}
 using nh::set_error;
 
 namespace {
@@ -116,41 +114,43 @@ int nh_net_view(const nh_net *net, nh_net_desc *d) {
 // (net-format/src/network.rs:654-692) at dt.
 int nh_net_bench(uint32_t n, uint32_t k, uint64_t seed, float w_scale, float w_inh, float dt_ms,
                  nh_net **out) {
-    if (!out) return set_error(NH_ERR_ARG, "null out");
-    std::unique_ptr<nh_net> net(new (std::nothrow) nh_net());
-    if (!net) return set_error(NH_ERR_CAPACITY, "out of host memory");
-    Rng rng(seed);
-    const uint64_t n_exc = (uint64_t)n * 4 / 5;
-    for (uint32_t i = 0; i < n; ++i) {
-        (void)rng.uniform(-500.0, 500.0);  // positions: drawn for stream parity, unused by
-        (void)rng.uniform(-500.0, 500.0);  // the bench (delays are set directly)
-        (void)rng.uniform(-500.0, 500.0);
-    }
-    net->n = n;
-    net->src_ptr.assign(n + 1, 0);
-    net->tgt.reserve((size_t)n * k);
-    net->w.reserve((size_t)n * k);
-    net->delay.reserve((size_t)n * k);
-    for (uint32_t src = 0; src < n; ++src) {
-        const bool exc = src < n_exc;
-        const float wm = exc ? 6.0f * w_scale : w_inh;
-        for (uint32_t q = 0; q < k; ++q) {
-            const uint32_t tgt = (uint32_t)rng.below(n);
-            const float delay_ms = exc ? (float)rng.uniform(1.0, 21.0) : 1.0f;
-            float steps = std::round(delay_ms / dt_ms);
-            if (steps < 1.0f) steps = 1.0f;
-            if (!std::isfinite(steps) || steps > 65535.0f) return set_error(NH_ERR_NET, "delay exceeds the 65535-step ring");
-            net->tgt.push_back(tgt);
-            net->w.push_back(exc ? std::fabs(wm) : -std::fabs(wm));
-            net->delay.push_back((uint16_t)steps);
+    return nh::guarded([&]() -> int {
+        if (!out) return set_error(NH_ERR_ARG, "null out");
+        std::unique_ptr<nh_net> net(new (std::nothrow) nh_net());
+        if (!net) return set_error(NH_ERR_CAPACITY, "out of host memory");
+        Rng rng(seed);
+        const uint64_t n_exc = (uint64_t)n * 4 / 5;
+        for (uint32_t i = 0; i < n; ++i) {
+            (void)rng.uniform(-500.0, 500.0);  // positions: drawn for stream parity, unused by
+            (void)rng.uniform(-500.0, 500.0);  // the bench (delays are set directly)
+            (void)rng.uniform(-500.0, 500.0);
         }
-        net->src_ptr[src + 1] = (uint32_t)net->tgt.size();
-    }
-    std::vector<uint8_t> inh(n);
-    for (uint32_t i = 0; i < n; ++i) inh[i] = i >= n_exc;
-    set_rows(*net, inh, 0);
-    *out = net.release();
-    return NH_OK;
+        net->n = n;
+        net->src_ptr.assign(n + 1, 0);
+        net->tgt.reserve((size_t)n * k);
+        net->w.reserve((size_t)n * k);
+        net->delay.reserve((size_t)n * k);
+        for (uint32_t src = 0; src < n; ++src) {
+            const bool exc = src < n_exc;
+            const float wm = exc ? 6.0f * w_scale : w_inh;
+            for (uint32_t q = 0; q < k; ++q) {
+                const uint32_t tgt = (uint32_t)rng.below(n);
+                const float delay_ms = exc ? (float)rng.uniform(1.0, 21.0) : 1.0f;
+                float steps = std::round(delay_ms / dt_ms);
+                if (steps < 1.0f) steps = 1.0f;
+                if (!std::isfinite(steps) || steps > 65535.0f) return set_error(NH_ERR_NET, "delay exceeds the 65535-step ring");
+                net->tgt.push_back(tgt);
+                net->w.push_back(exc ? std::fabs(wm) : -std::fabs(wm));
+                net->delay.push_back((uint16_t)steps);
+            }
+            net->src_ptr[src + 1] = (uint32_t)net->tgt.size();
+        }
+        std::vector<uint8_t> inh(n);
+        for (uint32_t i = 0; i < n; ++i) inh[i] = i >= n_exc;
+        set_rows(*net, inh, 0);
+        *out = net.release();
+        return NH_OK;
+    });
 }
 
 // Random-cube workload: positions U[-500,500)^3 (bench.rs:17-21 order),
@@ -161,55 +161,57 @@ int nh_net_bench(uint32_t n, uint32_t k, uint64_t seed, float w_scale, float w_i
 // clamp(1 + floor(19 d / dmax), 1, 20), d the f64 distance of the f32
 // positions, dmax = 1000*sqrt(3).
 int nh_net_cube(uint32_t n, double mean_deg, uint64_t seed, float w_scale, nh_net **out) {
-    if (!out) return set_error(NH_ERR_ARG, "null out");
-    if (n > (1u << 27)) return set_error(NH_ERR_ARG, "n too large");
-    std::unique_ptr<nh_net> net(new (std::nothrow) nh_net());
-    if (!net) return set_error(NH_ERR_CAPACITY, "out of host memory");
-    Rng rng(seed);
-    std::vector<float> pos((size_t)n * 3);
-    for (size_t i = 0; i < (size_t)n * 3; ++i) pos[i] = (float)rng.uniform(-500.0, 500.0);
-    const uint64_t n_exc = (uint64_t)n * 4 / 5;
-    const double p0 = std::exp(-mean_deg);
-    const double dmax = 1000.0 * std::sqrt(3.0);
-    net->n = n;
-    net->src_ptr.assign(n + 1, 0);
-    const size_t guess = (size_t)((double)n * mean_deg * 1.01) + 16;
-    net->tgt.reserve(guess);
-    net->w.reserve(guess);
-    net->delay.reserve(guess);
-    for (uint32_t s = 0; s < n; ++s) {
-        const double u = rng.f64();
-        uint32_t deg = 0;
-        double pk = p0, F = p0;
-        while (u > F && deg < 100000) {
-            ++deg;
-            pk *= mean_deg / (double)deg;
-            F += pk;
+    return nh::guarded([&]() -> int {
+        if (!out) return set_error(NH_ERR_ARG, "null out");
+        if (n > (1u << 27)) return set_error(NH_ERR_ARG, "n too large");
+        std::unique_ptr<nh_net> net(new (std::nothrow) nh_net());
+        if (!net) return set_error(NH_ERR_CAPACITY, "out of host memory");
+        Rng rng(seed);
+        std::vector<float> pos((size_t)n * 3);
+        for (size_t i = 0; i < (size_t)n * 3; ++i) pos[i] = (float)rng.uniform(-500.0, 500.0);
+        const uint64_t n_exc = (uint64_t)n * 4 / 5;
+        const double p0 = std::exp(-mean_deg);
+        const double dmax = 1000.0 * std::sqrt(3.0);
+        net->n = n;
+        net->src_ptr.assign(n + 1, 0);
+        const size_t guess = (size_t)((double)n * mean_deg * 1.01) + 16;
+        net->tgt.reserve(guess);
+        net->w.reserve(guess);
+        net->delay.reserve(guess);
+        for (uint32_t s = 0; s < n; ++s) {
+            const double u = rng.f64();
+            uint32_t deg = 0;
+            double pk = p0, F = p0;
+            while (u > F && deg < 100000) {
+                ++deg;
+                pk *= mean_deg / (double)deg;
+                F += pk;
+            }
+            const bool inh = s >= n_exc;
+            for (uint32_t q = 0; q < deg; ++q) {
+                uint32_t t;
+                do {
+                    t = (uint32_t)rng.below(n);
+                } while (t == s && n > 1);
+                const float wm = rng.f32() * w_scale;
+                const double dx = (double)pos[3 * (size_t)s] - pos[3 * (size_t)t];
+                const double dy = (double)pos[3 * (size_t)s + 1] - pos[3 * (size_t)t + 1];
+                const double dz = (double)pos[3 * (size_t)s + 2] - pos[3 * (size_t)t + 2];
+                const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+                int dl = 1 + (int)std::floor(19.0 * d / dmax);
+                dl = std::min(std::max(dl, 1), 20);
+                net->tgt.push_back(t);
+                net->w.push_back(inh ? -std::fabs(wm) : std::fabs(wm));
+                net->delay.push_back((uint16_t)dl);
+            }
+            net->src_ptr[s + 1] = (uint32_t)net->tgt.size();
         }
-        const bool inh = s >= n_exc;
-        for (uint32_t q = 0; q < deg; ++q) {
-            uint32_t t;
-            do {
-                t = (uint32_t)rng.below(n);
-            } while (t == s && n > 1);
-            const float wm = rng.f32() * w_scale;
-            const double dx = (double)pos[3 * (size_t)s] - pos[3 * (size_t)t];
-            const double dy = (double)pos[3 * (size_t)s + 1] - pos[3 * (size_t)t + 1];
-            const double dz = (double)pos[3 * (size_t)s + 2] - pos[3 * (size_t)t + 2];
-            const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
-            int dl = 1 + (int)std::floor(19.0 * d / dmax);
-            dl = std::min(std::max(dl, 1), 20);
-            net->tgt.push_back(t);
-            net->w.push_back(inh ? -std::fabs(wm) : std::fabs(wm));
-            net->delay.push_back((uint16_t)dl);
-        }
-        net->src_ptr[s + 1] = (uint32_t)net->tgt.size();
-    }
-    std::vector<uint8_t> inhv(n);
-    for (uint32_t i = 0; i < n; ++i) inhv[i] = i >= n_exc;
-    set_rows(*net, inhv, 0);
-    *out = net.release();
-    return NH_OK;
+        std::vector<uint8_t> inhv(n);
+        for (uint32_t i = 0; i < n; ++i) inhv[i] = i >= n_exc;
+        set_rows(*net, inhv, 0);
+        *out = net.release();
+        return NH_OK;
+    });
 }
 
 // 3D-grid liquid.  Lattice as builders::create_3d_grid (builders.rs:88-131)
@@ -228,105 +230,107 @@ int nh_net_cube(uint32_t n, double mean_deg, uint64_t seed, float w_scale, nh_ne
 int nh_net_grid(uint32_t ex, uint32_t ey, uint32_t ez, double scale, double lambda, double inh_frac,
                 float w_exc, float w_inh, int model, uint32_t n_inputs, uint32_t per_input, float w_in,
                 uint64_t seed, float dt_ms, nh_net **out) {
-    if (!out) return set_error(NH_ERR_ARG, "null out");
-    const uint64_t n64 = (uint64_t)ex * ey * ez;
-    if (n64 == 0 || n64 > (1u << 26)) return set_error(NH_ERR_ARG, "bad grid size");
-    std::unique_ptr<nh_net> net(new (std::nothrow) nh_net());
-    if (!net) return set_error(NH_ERR_CAPACITY, "out of host memory");
-    const uint32_t n = (uint32_t)n64;
-    const float spacing = 60.0f, speed = 100.0f;
-    Rng rng(seed);
-    // lattice in create_3d_grid order: z outer, x, y inner
-    std::vector<uint32_t> gx(n), gy(n), gz(n);
-    std::vector<float> pos((size_t)n * 3);
-    {
-        uint32_t id = 0;
-        for (uint32_t z = 0; z < ez; ++z)
-            for (uint32_t x = 0; x < ex; ++x)
-                for (uint32_t y = 0; y < ey; ++y) {
-                    gx[id] = x;
-                    gy[id] = y;
-                    gz[id] = z;
-                    pos[3 * (size_t)id] = ((float)x - (float)ex / 2.0f) * spacing;
-                    pos[3 * (size_t)id + 1] = ((float)y - (float)ey / 2.0f) * spacing;
-                    pos[3 * (size_t)id + 2] = ((float)z - (float)ez / 2.0f) * spacing;
-                    ++id;
+    return nh::guarded([&]() -> int {
+        if (!out) return set_error(NH_ERR_ARG, "null out");
+        const uint64_t n64 = (uint64_t)ex * ey * ez;
+        if (n64 == 0 || n64 > (1u << 26)) return set_error(NH_ERR_ARG, "bad grid size");
+        std::unique_ptr<nh_net> net(new (std::nothrow) nh_net());
+        if (!net) return set_error(NH_ERR_CAPACITY, "out of host memory");
+        const uint32_t n = (uint32_t)n64;
+        const float spacing = 60.0f, speed = 100.0f;
+        Rng rng(seed);
+        // lattice in create_3d_grid order: z outer, x, y inner
+        std::vector<uint32_t> gx(n), gy(n), gz(n);
+        std::vector<float> pos((size_t)n * 3);
+        {
+            uint32_t id = 0;
+            for (uint32_t z = 0; z < ez; ++z)
+                for (uint32_t x = 0; x < ex; ++x)
+                    for (uint32_t y = 0; y < ey; ++y) {
+                        gx[id] = x;
+                        gy[id] = y;
+                        gz[id] = z;
+                        pos[3 * (size_t)id] = ((float)x - (float)ex / 2.0f) * spacing;
+                        pos[3 * (size_t)id + 1] = ((float)y - (float)ey / 2.0f) * spacing;
+                        pos[3 * (size_t)id + 2] = ((float)z - (float)ez / 2.0f) * spacing;
+                        ++id;
+                    }
+        }
+        auto id_of = [&](uint32_t x, uint32_t y, uint32_t z) { return (z * ex + x) * ey + y; };
+        std::vector<uint8_t> inh(n);
+        for (uint32_t i = 0; i < n; ++i) inh[i] = rng.f64() < inh_frac ? 1 : 0;
+        // offsets within the cutoff, with their probability and delay
+        const double lam = lambda * spacing;
+        const int R = (int)std::ceil(3.0 * lambda);
+        struct Off {
+            int dx, dy, dz;
+            double p;
+            uint16_t delay;
+        };
+        std::vector<Off> offs;
+        for (int dz = -R; dz <= R; ++dz)
+            for (int dx = -R; dx <= R; ++dx)
+                for (int dy = -R; dy <= R; ++dy) {
+                    if (!dx && !dy && !dz) continue;
+                    const float fx = (float)dx * spacing, fy = (float)dy * spacing, fz = (float)dz * spacing;
+                    const float df = std::sqrt(fx * fx + fy * fy + fz * fz);
+                    const double d = (double)df;
+                    if (d > 3.0 * lam) continue;
+                    const double q = d / lam;
+                    const double p = scale * std::exp(-(q * q));
+                    float dms = df / speed;
+                    if (dms < 1e-3f) dms = 1e-3f;
+                    float steps = std::round(dms / dt_ms);
+                    if (steps < 1.0f) steps = 1.0f;
+                    if (steps > 65535.0f) return set_error(NH_ERR_NET, "delay exceeds the 65535-step ring");
+                    offs.push_back({dx, dy, dz, p, (uint16_t)steps});
                 }
-    }
-    auto id_of = [&](uint32_t x, uint32_t y, uint32_t z) { return (z * ex + x) * ey + y; };
-    std::vector<uint8_t> inh(n);
-    for (uint32_t i = 0; i < n; ++i) inh[i] = rng.f64() < inh_frac ? 1 : 0;
-    // offsets within the cutoff, with their probability and delay
-    const double lam = lambda * spacing;
-    const int R = (int)std::ceil(3.0 * lambda);
-    struct Off {
-        int dx, dy, dz;
-        double p;
-        uint16_t delay;
-    };
-    std::vector<Off> offs;
-    for (int dz = -R; dz <= R; ++dz)
-        for (int dx = -R; dx <= R; ++dx)
-            for (int dy = -R; dy <= R; ++dy) {
-                if (!dx && !dy && !dz) continue;
-                const float fx = (float)dx * spacing, fy = (float)dy * spacing, fz = (float)dz * spacing;
-                const float df = std::sqrt(fx * fx + fy * fy + fz * fz);
-                const double d = (double)df;
-                if (d > 3.0 * lam) continue;
-                const double q = d / lam;
-                const double p = scale * std::exp(-(q * q));
-                float dms = df / speed;
-                if (dms < 1e-3f) dms = 1e-3f;
-                float steps = std::round(dms / dt_ms);
-                if (steps < 1.0f) steps = 1.0f;
-                if (steps > 65535.0f) return set_error(NH_ERR_NET, "delay exceeds the 65535-step ring");
-                offs.push_back({dx, dy, dz, p, (uint16_t)steps});
+        const float wmul = model == 1 ? kTonic[0] : 1.0f;
+        net->n = n;
+        net->src_ptr.assign(n + 1, 0);
+        for (uint32_t s = 0; s < n; ++s) {
+            for (const Off &o : offs) {
+                const int x = (int)gx[s] + o.dx, y = (int)gy[s] + o.dy, z = (int)gz[s] + o.dz;
+                if (x < 0 || y < 0 || z < 0 || x >= (int)ex || y >= (int)ey || z >= (int)ez) continue;
+                if (rng.f64() < o.p) {
+                    const float wm = (inh[s] ? w_inh : w_exc) * wmul;
+                    net->tgt.push_back(id_of((uint32_t)x, (uint32_t)y, (uint32_t)z));
+                    net->w.push_back(inh[s] ? -std::fabs(wm) : std::fabs(wm));
+                    net->delay.push_back(o.delay);
+                }
             }
-    const float wmul = model == 1 ? kTonic[0] : 1.0f;
-    net->n = n;
-    net->src_ptr.assign(n + 1, 0);
-    for (uint32_t s = 0; s < n; ++s) {
-        for (const Off &o : offs) {
-            const int x = (int)gx[s] + o.dx, y = (int)gy[s] + o.dy, z = (int)gz[s] + o.dz;
-            if (x < 0 || y < 0 || z < 0 || x >= (int)ex || y >= (int)ey || z >= (int)ez) continue;
-            if (rng.f64() < o.p) {
-                const float wm = (inh[s] ? w_inh : w_exc) * wmul;
-                net->tgt.push_back(id_of((uint32_t)x, (uint32_t)y, (uint32_t)z));
-                net->w.push_back(inh[s] ? -std::fabs(wm) : std::fabs(wm));
-                net->delay.push_back(o.delay);
+            net->src_ptr[s + 1] = (uint32_t)net->tgt.size();
+        }
+        set_rows(*net, inh, model);
+        // inputs
+        net->n_inputs = n_inputs;
+        if (n_inputs) {
+            std::vector<std::vector<uint32_t>> tg(n_inputs);
+            const uint32_t shared = per_input * 2;
+            for (uint32_t p0 = 0; p0 < n_inputs; p0 += 2) {
+                std::vector<uint32_t> pool(shared);
+                for (auto &m : pool) m = (uint32_t)rng.below(n);
+                for (uint32_t sidx = p0; sidx < std::min(p0 + 2, n_inputs); ++sidx) {
+                    tg[sidx] = pool;
+                    for (uint32_t q = 0; q < per_input; ++q) tg[sidx].push_back((uint32_t)rng.below(n));
+                }
+            }
+            // StimMap::from_network (sim/src/stim.rs:34-57)
+            std::vector<std::vector<std::pair<uint32_t, float>>> per(n);
+            for (uint32_t k = 0; k < n_inputs; ++k)
+                for (uint32_t m : tg[k]) per[m].push_back({k, w_in * wmul});
+            net->stim_ptr.assign(n + 1, 0);
+            for (uint32_t i = 0; i < n; ++i) {
+                net->stim_ptr[i + 1] = net->stim_ptr[i] + (uint32_t)per[i].size();
+                for (auto &e : per[i]) {
+                    net->stim_input.push_back(e.first);
+                    net->stim_w.push_back(e.second);
+                }
             }
         }
-        net->src_ptr[s + 1] = (uint32_t)net->tgt.size();
-    }
-    set_rows(*net, inh, model);
-    // inputs
-    net->n_inputs = n_inputs;
-    if (n_inputs) {
-        std::vector<std::vector<uint32_t>> tg(n_inputs);
-        const uint32_t shared = per_input * 2;
-        for (uint32_t p0 = 0; p0 < n_inputs; p0 += 2) {
-            std::vector<uint32_t> pool(shared);
-            for (auto &m : pool) m = (uint32_t)rng.below(n);
-            for (uint32_t sidx = p0; sidx < std::min(p0 + 2, n_inputs); ++sidx) {
-                tg[sidx] = pool;
-                for (uint32_t q = 0; q < per_input; ++q) tg[sidx].push_back((uint32_t)rng.below(n));
-            }
-        }
-        // StimMap::from_network (sim/src/stim.rs:34-57)
-        std::vector<std::vector<std::pair<uint32_t, float>>> per(n);
-        for (uint32_t k = 0; k < n_inputs; ++k)
-            for (uint32_t m : tg[k]) per[m].push_back({k, w_in * wmul});
-        net->stim_ptr.assign(n + 1, 0);
-        for (uint32_t i = 0; i < n; ++i) {
-            net->stim_ptr[i + 1] = net->stim_ptr[i] + (uint32_t)per[i].size();
-            for (auto &e : per[i]) {
-                net->stim_input.push_back(e.first);
-                net->stim_w.push_back(e.second);
-            }
-        }
-    }
-    *out = net.release();
-    return NH_OK;
+        *out = net.release();
+        return NH_OK;
+    });
 }
 
 struct nh_rng_stream {
@@ -337,9 +341,11 @@ struct nh_rng_stream {
 };
 
 int nh_rng_stream_new(uint64_t seed, uint32_t n_src, float p, nh_rng_stream **out) {
-    if (!out) return set_error(NH_ERR_ARG, "null out");
-    *out = new (std::nothrow) nh_rng_stream(seed, n_src, p);
-    return *out ? NH_OK : set_error(NH_ERR_CAPACITY, "out of host memory");
+    return nh::guarded([&]() -> int {
+        if (!out) return set_error(NH_ERR_ARG, "null out");
+        *out = new (std::nothrow) nh_rng_stream(seed, n_src, p);
+        return *out ? NH_OK : set_error(NH_ERR_CAPACITY, "out of host memory");
+    });
 }
 
 void nh_rng_stream_free(nh_rng_stream *s) { delete s; }
@@ -347,24 +353,26 @@ void nh_rng_stream_free(nh_rng_stream *s) { delete s; }
 // RngStream::window -- sim/src/script.rs:2867-2894
 int nh_rng_stream_window(nh_rng_stream *s, uint32_t steps, int event, const uint32_t *pairs,
                          uint32_t n_pairs, float strength, float *out) {
-    if (!s || (!out && steps)) return set_error(NH_ERR_ARG, "null argument");
-    const double p = (double)s->p;
-    for (uint32_t t = 0; t < steps; ++t) {
-        float *row = out + (size_t)t * s->n_src;
-        for (uint32_t k = 0; k < s->n_src; ++k) row[k] = s->rng.f64() < p ? 1.0f : 0.0f;
-        if (event && strength > 0.0f) {
-            for (uint32_t q = 0; q < n_pairs; ++q) {
-                const uint32_t i = pairs[2 * q], j = pairs[2 * q + 1];
-                if (i >= s->n_src || j >= s->n_src) continue;
-                if (s->rng.f64() < (double)strength) {
-                    const float bit = s->rng.f64() < p ? 1.0f : 0.0f;
-                    row[i] = bit;
-                    row[j] = bit;
+    return nh::guarded([&]() -> int {
+        if (!s || (!out && steps)) return set_error(NH_ERR_ARG, "null argument");
+        const double p = (double)s->p;
+        for (uint32_t t = 0; t < steps; ++t) {
+            float *row = out + (size_t)t * s->n_src;
+            for (uint32_t k = 0; k < s->n_src; ++k) row[k] = s->rng.f64() < p ? 1.0f : 0.0f;
+            if (event && strength > 0.0f) {
+                for (uint32_t q = 0; q < n_pairs; ++q) {
+                    const uint32_t i = pairs[2 * q], j = pairs[2 * q + 1];
+                    if (i >= s->n_src || j >= s->n_src) continue;
+                    if (s->rng.f64() < (double)strength) {
+                        const float bit = s->rng.f64() < p ? 1.0f : 0.0f;
+                        row[i] = bit;
+                        row[j] = bit;
+                    }
                 }
             }
         }
-    }
-    return NH_OK;
+        return NH_OK;
+    });
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 // sees only its own local step (noise hash, stimulus, windows), so its
 // result equals the plain path's result for it alone.
 #include <algorithm>
+#include <atomic>
 #include <cassert>
 #include <chrono>
 #include <cmath>
@@ -597,32 +598,34 @@ void nh_server_config_default(nh_server_config *c) {
 }
 
 int nh_server_create(const nh_server_config *cfg, nh_server **out) {
-    if (!cfg || !out) return set_error(NH_ERR_ARG, "null argument");
-    *out = nullptr;
-    int dev = 0;
-    if (hipGetDeviceCount(&dev) != hipSuccess || dev < 1) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
-    if (cfg->max_delay < 1 || cfg->max_delay > 31) return set_error(NH_ERR_ARG, "max_delay must be 1..31");
-    if (cfg->max_requests < 1 || cfg->max_requests > 65535) return set_error(NH_ERR_ARG, "max_requests");
-    std::unique_ptr<nh_server> s(new (std::nothrow) nh_server());
-    if (!s) return set_error(NH_ERR_CAPACITY, "out of host memory");
-    s->cfg = *cfg;
-    nh_server_config &c = s->cfg;
-    if (c.chunk_steps == 0) c.chunk_steps = 100;
-    // admission by free HBM: capacities left 0 come from the free memory
-    size_t fr = 0, tot = 0;
-    HIPCHK(hipMemGetInfo(&fr, &tot));
-    const double budget = (double)fr * (c.hbm_fraction > 0.0f ? c.hbm_fraction : 0.25f);
-    const double per_neuron = 8 + 4 + 4.0 * (c.max_delay + 1) + 1 + 8 + 8 + 8 + 4.0 * c.n_windows +
-                              (c.n_inputs ? 8 + 8 * 8 : 0);
-    if (!c.cap_neurons) c.cap_neurons = std::min<uint64_t>((uint64_t)(budget * 0.5 / per_neuron), 16ull << 20);
-    if (!c.cap_synapses) c.cap_synapses = std::min<uint64_t>((uint64_t)(budget * 0.5 / 8.0), 1ull << 30);
-    c.cap_neurons = std::min<uint64_t>(c.cap_neurons, 1ull << 27) & ~uint64_t(63);
-    if (c.cap_neurons < 64) return set_error(NH_ERR_CAPACITY, "not enough HBM for the arena");
-    if (!c.cap_stim) c.cap_stim = c.n_inputs ? c.cap_neurons : 0;
-    TRY(build_arena(s.get()));
-    s->worker = std::thread(worker_loop, s.get());
-    *out = s.release();
-    return NH_OK;
+    return nh::guarded([&]() -> int {
+        if (!cfg || !out) return set_error(NH_ERR_ARG, "null argument");
+        *out = nullptr;
+        int dev = 0;
+        if (hipGetDeviceCount(&dev) != hipSuccess || dev < 1) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
+        if (cfg->max_delay < 1 || cfg->max_delay > 31) return set_error(NH_ERR_ARG, "max_delay must be 1..31");
+        if (cfg->max_requests < 1 || cfg->max_requests > 65535) return set_error(NH_ERR_ARG, "max_requests");
+        std::unique_ptr<nh_server> s(new (std::nothrow) nh_server());
+        if (!s) return set_error(NH_ERR_CAPACITY, "out of host memory");
+        s->cfg = *cfg;
+        nh_server_config &c = s->cfg;
+        if (c.chunk_steps == 0) c.chunk_steps = 100;
+        // admission by free HBM: capacities left 0 come from the free memory
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const double budget = (double)fr * (c.hbm_fraction > 0.0f ? c.hbm_fraction : 0.25f);
+        const double per_neuron = 8 + 4 + 4.0 * (c.max_delay + 1) + 1 + 8 + 8 + 8 + 4.0 * c.n_windows +
+                                  (c.n_inputs ? 8 + 8 * 8 : 0);
+        if (!c.cap_neurons) c.cap_neurons = std::min<uint64_t>((uint64_t)(budget * 0.5 / per_neuron), 16ull << 20);
+        if (!c.cap_synapses) c.cap_synapses = std::min<uint64_t>((uint64_t)(budget * 0.5 / 8.0), 1ull << 30);
+        c.cap_neurons = std::min<uint64_t>(c.cap_neurons, 1ull << 27) & ~uint64_t(63);
+        if (c.cap_neurons < 64) return set_error(NH_ERR_CAPACITY, "not enough HBM for the arena");
+        if (!c.cap_stim) c.cap_stim = c.n_inputs ? c.cap_neurons : 0;
+        TRY(build_arena(s.get()));
+        s->worker = std::thread(worker_loop, s.get());
+        *out = s.release();
+        return NH_OK;
+    });
 }
 
 void nh_server_destroy(nh_server *s) {
@@ -640,90 +643,93 @@ void nh_server_destroy(nh_server *s) {
 
 int nh_server_submit(nh_server *s, const nh_net_desc *d, const nh_config *cfg, const float *stim,
                      uint64_t stim_steps, uint64_t total_steps, uint32_t win_steps, uint64_t *id) {
-    if (!s || !d || !cfg || !id) return set_error(NH_ERR_ARG, "null argument");
-    TRY(nh_check_net(d, cfg));
-    if ((cfg->int_weights ? 1 : 0) != s->cfg.int_weights)
-        return set_error(NH_ERR_ARG, "request int_weights differs from the server's ring");
-    if (d->n_inputs > s->cfg.n_inputs) return set_error(NH_ERR_ARG, "too many inputs for the server");
-    if (d->n && d->n_state != 2) return set_error(NH_ERR_ARG, "state row width must be 2");
-    if (d->n_params > (uint32_t)nh::kMaxParams) return set_error(NH_ERR_ARG, "param row too wide");
-    if (total_steps == 0 || total_steps > 0xFFFFFFFFull)
-        return set_error(NH_ERR_ARG, "total_steps must be 1..2^32-1");
-    auto r = std::make_unique<Request>();
-    r->submit_ns = now_ns();  // latency counts the request's preparation too
-    r->n = d->n;
-    r->n_al = (d->n + 63) & ~63u;
-    r->cfg = *cfg;
-    r->total_steps = total_steps;
-    r->win_steps = win_steps;
-    r->E = d->n ? d->src_ptr[d->n] : 0;
-    if (r->n_al / 64 > s->cfg.cap_neurons / 64 || r->E > s->cfg.cap_synapses)
-        return set_error(NH_ERR_CAPACITY, "request larger than the server arena");
-    // rows sorted by (delay, target), local targets packed with the delay
-    // (src_ptr is checked monotone from 0 by nh_check_net); large requests
-    // split their rows over a few threads: the preparation is part of the
-    // request's latency
-    if (r->n) r->row_off.assign(d->src_ptr, d->src_ptr + r->n + 1);
-    else r->row_off.assign(1, 0);
-    r->syn.resize(r->E);
-    if (s->cfg.int_weights) r->wq.resize(r->E);
-    else r->wf.resize(r->E);
-    {
-        const bool iw = s->cfg.int_weights != 0;
-        const uint32_t nt = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(8, r->E / 65536 + 1), r->n ? r->n : 1);
-        std::vector<const char *> err(nt, nullptr);
-        std::vector<std::thread> pool;
-        for (uint32_t q = 1; q < nt; ++q)
-            pool.emplace_back([&, q] {
-                err[q] = prep_rows(d, cfg, s->cfg.max_delay, iw, (uint32_t)((uint64_t)r->n * q / nt),
-                                   (uint32_t)((uint64_t)r->n * (q + 1) / nt), r.get());
-            });
-        err[0] = prep_rows(d, cfg, s->cfg.max_delay, iw, 0, (uint32_t)(r->n / nt), r.get());
-        for (auto &th : pool) th.join();
-        for (const char *e : err)
-            if (e) return set_error(std::strstr(e, "custom-tau") ? NH_ERR_UNSUPPORTED : NH_ERR_ARG, e);
-    }
-    // distinct parameter rows
-    r->row_of.resize(r->n);
-    for (uint32_t i = 0; i < r->n; ++i) {
-        if (d->model[i] >= d->n_models || d->model_ids[d->model[i]] > 1)
-            return set_error(NH_ERR_ARG, "bad model index");
-        nh::PRow pr{};
-        pr.model = d->model_ids[d->model[i]];
-        for (uint32_t q = 0; q < d->n_params; ++q) pr.p[q] = d->p[(uint64_t)i * d->n_params + q];
-        size_t k = 0;
-        for (; k < r->rows.size(); ++k)
-            if (std::memcmp(&r->rows[k], &pr, sizeof(pr)) == 0) break;
-        if (k == r->rows.size()) {
-            if (k >= 256) return set_error(NH_ERR_CAPACITY, "too many distinct parameter rows");
-            r->rows.push_back(pr);
+    return nh::guarded([&]() -> int {
+        if (!s || !d || !cfg || !id) return set_error(NH_ERR_ARG, "null argument");
+        TRY(nh_check_net(d, cfg));
+        if ((cfg->int_weights ? 1 : 0) != s->cfg.int_weights)
+            return set_error(NH_ERR_ARG, "request int_weights differs from the server's ring");
+        if (d->n_inputs > s->cfg.n_inputs) return set_error(NH_ERR_ARG, "too many inputs for the server");
+        if (d->n && d->n_state != 2) return set_error(NH_ERR_ARG, "state row width must be 2");
+        if (d->n_params > (uint32_t)nh::kMaxParams) return set_error(NH_ERR_ARG, "param row too wide");
+        if (total_steps == 0 || total_steps > 0xFFFFFFFFull)
+            return set_error(NH_ERR_ARG, "total_steps must be 1..2^32-1");
+        auto r = std::make_unique<Request>();
+        r->submit_ns = now_ns();  // latency counts the request's preparation too
+        r->n = d->n;
+        r->n_al = (d->n + 63) & ~63u;
+        r->cfg = *cfg;
+        r->total_steps = total_steps;
+        r->win_steps = win_steps;
+        r->E = d->n ? d->src_ptr[d->n] : 0;
+        if (r->n_al / 64 > s->cfg.cap_neurons / 64 || r->E > s->cfg.cap_synapses)
+            return set_error(NH_ERR_CAPACITY, "request larger than the server arena");
+        // rows sorted by (delay, target), local targets packed with the delay
+        // (src_ptr is checked monotone from 0 by nh_check_net); large requests
+        // split their rows over a few threads: the preparation is part of the
+        // request's latency
+        if (r->n) r->row_off.assign(d->src_ptr, d->src_ptr + r->n + 1);
+        else r->row_off.assign(1, 0);
+        r->syn.resize(r->E);
+        if (s->cfg.int_weights) r->wq.resize(r->E);
+        else r->wf.resize(r->E);
+        {
+            const bool iw = s->cfg.int_weights != 0;
+            // chunks of about 32k synapses (at most 8): small requests stay on
+            // the calling thread
+            const uint32_t nt = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(8, r->E / 32768 + 1), r->n ? r->n : 1);
+            std::vector<const char *> err(nt, nullptr);
+            std::atomic<uint32_t> next{0};
+            auto work = [&] {
+                for (uint32_t q; (q = next.fetch_add(1)) < nt;)
+                    err[q] = prep_rows(d, cfg, s->cfg.max_delay, iw, (uint32_t)((uint64_t)r->n * q / nt),
+                                       (uint32_t)((uint64_t)r->n * (q + 1) / nt), r.get());
+            };
+            nh::run_workers(work, nt);
+            for (const char *e : err)
+                if (e) return set_error(std::strstr(e, "custom-tau") ? NH_ERR_UNSUPPORTED : NH_ERR_ARG, e);
         }
-        r->row_of[i] = (uint8_t)k;
-    }
-    r->x0.resize(r->n);
-    for (uint32_t i = 0; i < r->n; ++i) r->x0[i] = make_float2(d->x0[2 * (uint64_t)i], d->x0[2 * (uint64_t)i + 1]);
-    r->stim_off.assign(r->n + 1, 0);
-    if (d->n_inputs && d->stim_ptr) {
-        for (uint32_t i = 0; i <= r->n; ++i) r->stim_off[i] = d->stim_ptr[i];
-        r->n_stim = d->stim_ptr[r->n];
-        r->stim_in.assign(d->stim_input, d->stim_input + r->n_stim);
-        r->stim_w.assign(d->stim_w, d->stim_w + r->n_stim);
-    }
-    if (s->cfg.n_inputs) {
-        r->stim_steps = stim_steps;
-        r->stim.assign(stim_steps * s->cfg.n_inputs, 0.0f);
-        for (uint64_t t = 0; t < stim_steps; ++t)
-            for (uint32_t k = 0; k < d->n_inputs; ++k)
-                r->stim[t * s->cfg.n_inputs + k] = stim[t * d->n_inputs + k];
-    }
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->worker_error) return set_error(s->worker_error, "server stopped: " + s->worker_msg);
-    r->id = s->next_id++;
-    *id = r->id;
-    s->queue.push_back(r.get());
-    s->all.emplace(r->id, std::move(r));
-    s->cv_submit.notify_all();
-    return NH_OK;
+        // distinct parameter rows
+        r->row_of.resize(r->n);
+        for (uint32_t i = 0; i < r->n; ++i) {
+            if (d->model[i] >= d->n_models || d->model_ids[d->model[i]] > 1)
+                return set_error(NH_ERR_ARG, "bad model index");
+            nh::PRow pr{};
+            pr.model = d->model_ids[d->model[i]];
+            for (uint32_t q = 0; q < d->n_params; ++q) pr.p[q] = d->p[(uint64_t)i * d->n_params + q];
+            size_t k = 0;
+            for (; k < r->rows.size(); ++k)
+                if (std::memcmp(&r->rows[k], &pr, sizeof(pr)) == 0) break;
+            if (k == r->rows.size()) {
+                if (k >= 256) return set_error(NH_ERR_CAPACITY, "too many distinct parameter rows");
+                r->rows.push_back(pr);
+            }
+            r->row_of[i] = (uint8_t)k;
+        }
+        r->x0.resize(r->n);
+        for (uint32_t i = 0; i < r->n; ++i) r->x0[i] = make_float2(d->x0[2 * (uint64_t)i], d->x0[2 * (uint64_t)i + 1]);
+        r->stim_off.assign(r->n + 1, 0);
+        if (d->n_inputs && d->stim_ptr) {
+            for (uint32_t i = 0; i <= r->n; ++i) r->stim_off[i] = d->stim_ptr[i];
+            r->n_stim = d->stim_ptr[r->n];
+            r->stim_in.assign(d->stim_input, d->stim_input + r->n_stim);
+            r->stim_w.assign(d->stim_w, d->stim_w + r->n_stim);
+        }
+        if (s->cfg.n_inputs) {
+            r->stim_steps = stim_steps;
+            r->stim.assign(stim_steps * s->cfg.n_inputs, 0.0f);
+            for (uint64_t t = 0; t < stim_steps; ++t)
+                for (uint32_t k = 0; k < d->n_inputs; ++k)
+                    r->stim[t * s->cfg.n_inputs + k] = stim[t * d->n_inputs + k];
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->worker_error) return set_error(s->worker_error, "server stopped: " + s->worker_msg);
+        r->id = s->next_id++;
+        *id = r->id;
+        s->queue.push_back(r.get());
+        s->all.emplace(r->id, std::move(r));
+        s->cv_submit.notify_all();
+        return NH_OK;
+    });
 }
 
 int nh_server_wait(nh_server *s, uint64_t id, uint32_t timeout_ms, uint32_t *counts, uint64_t cap,

@@ -127,40 +127,44 @@ int nh_check_net(const nh_net_desc *net, const nh_config *cfg) {
 }
 
 int nh_create(const nh_net_desc *net, const nh_config *cfg, nh_sim **out) {
-    if (!net || !cfg || !out) return set_error(NH_ERR_ARG, "null argument");
-    *out = nullptr;
-    TRY(nh_check_net(net, cfg));
-    if (!nh_available()) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
-    std::unique_ptr<nh_sim> s(new (std::nothrow) nh_sim());
-    if (!s) return set_error(NH_ERR_CAPACITY, "out of host memory");
-    s->cfg = *cfg;
-    s->n = net->n;
-    s->n_inputs = net->n_inputs;
-    nh::Tuning tune;
-    tune.fused = cfg->step_shape;
-    tune.use_graph = cfg->exec_mode == 0 ? -1 : (cfg->exec_mode == 2 ? 1 : 0);
-    tune.graph_steps = cfg->graph_steps ? cfg->graph_steps : 64;
-    tune.block = cfg->block;
-    TRY(s->ar.build(1, net, cfg, cfg->raster_steps, tune));
-    s->constants.assign(s->n_inputs, 0.0f);
-    s->series.assign(s->n_inputs, {});
-    s->play_start.assign(s->n_inputs, 0);
-    s->looped.assign(s->n_inputs, 0);
-    s->ext_host.assign(s->n, 0.0f);
-    s->ar.set_stim_packer(pack_stim, s.get());
-    const float rt = cfg->readout_tau_ms > 1e-6f ? cfg->readout_tau_ms : 1e-6f;
-    s->decay_t = std::exp(-cfg->dt_ms / rt);  // cpu.rs:204
-    *out = s.release();
-    return NH_OK;
+    return nh::guarded([&]() -> int {
+        if (!net || !cfg || !out) return set_error(NH_ERR_ARG, "null argument");
+        *out = nullptr;
+        TRY(nh_check_net(net, cfg));
+        if (!nh_available()) return set_error(NH_ERR_NO_DEVICE, "no HIP device");
+        std::unique_ptr<nh_sim> s(new (std::nothrow) nh_sim());
+        if (!s) return set_error(NH_ERR_CAPACITY, "out of host memory");
+        s->cfg = *cfg;
+        s->n = net->n;
+        s->n_inputs = net->n_inputs;
+        nh::Tuning tune;
+        tune.fused = cfg->step_shape;
+        tune.use_graph = cfg->exec_mode == 0 ? -1 : (cfg->exec_mode == 2 ? 1 : 0);
+        tune.graph_steps = cfg->graph_steps ? cfg->graph_steps : 64;
+        tune.block = cfg->block;
+        TRY(s->ar.build(1, net, cfg, cfg->raster_steps, tune));
+        s->constants.assign(s->n_inputs, 0.0f);
+        s->series.assign(s->n_inputs, {});
+        s->play_start.assign(s->n_inputs, 0);
+        s->looped.assign(s->n_inputs, 0);
+        s->ext_host.assign(s->n, 0.0f);
+        s->ar.set_stim_packer(pack_stim, s.get());
+        const float rt = cfg->readout_tau_ms > 1e-6f ? cfg->readout_tau_ms : 1e-6f;
+        s->decay_t = std::exp(-cfg->dt_ms / rt);  // cpu.rs:204
+        *out = s.release();
+        return NH_OK;
+    });
 }
 
 void nh_destroy(nh_sim *s) { delete s; }
 
 int nh_run_steps(nh_sim *s, uint64_t n_steps) {
-    CHECK_SIM(s);
-    TRY(s->ar.run(n_steps));
-    if (!s->v_watch.empty()) s->vrec_valid += n_steps;
-    return NH_OK;
+    return nh::guarded([&]() -> int {
+        CHECK_SIM(s);
+        TRY(s->ar.run(n_steps));
+        if (!s->v_watch.empty()) s->vrec_valid += n_steps;
+        return NH_OK;
+    });
 }
 
 int nh_run_ms(nh_sim *s, float ms) {
@@ -635,21 +639,23 @@ int nh_grid_traces(nh_sim *s, float tau_ms, float *out, uint64_t cap) {
 
 int nh_read_spikes(nh_sim *s, uint64_t *step0, uint32_t *counts, uint64_t max_steps, uint32_t *ids,
                    uint64_t max_ids, uint64_t *n_steps, uint64_t *n_ids) {
-    CHECK_SIM(s);
-    auto &ar = s->ar;
-    if (!ar.rast_steps) return set_error(NH_ERR_UNSUPPORTED, "raster stream off (raster_steps = 0)");
-    TRY(ar.drain_raster());
-    *n_steps = ar.h_rast_cnt.size();
-    *n_ids = ar.h_rast_ids.size();
-    *step0 = ar.h_rast_step0;
-    if (*n_steps > max_steps || *n_ids > max_ids)
-        return set_error(NH_ERR_CAPACITY, "raster output arrays too small");
-    std::copy(ar.h_rast_cnt.begin(), ar.h_rast_cnt.end(), counts);
-    std::copy(ar.h_rast_ids.begin(), ar.h_rast_ids.end(), ids);
-    ar.h_rast_cnt.clear();
-    ar.h_rast_ids.clear();
-    ar.h_rast_step0 = ar.t;
-    return NH_OK;
+    return nh::guarded([&]() -> int {
+        CHECK_SIM(s);
+        auto &ar = s->ar;
+        if (!ar.rast_steps) return set_error(NH_ERR_UNSUPPORTED, "raster stream off (raster_steps = 0)");
+        TRY(ar.drain_raster());
+        *n_steps = ar.h_rast_cnt.size();
+        *n_ids = ar.h_rast_ids.size();
+        *step0 = ar.h_rast_step0;
+        if (*n_steps > max_steps || *n_ids > max_ids)
+            return set_error(NH_ERR_CAPACITY, "raster output arrays too small");
+        std::copy(ar.h_rast_cnt.begin(), ar.h_rast_cnt.end(), counts);
+        std::copy(ar.h_rast_ids.begin(), ar.h_rast_ids.end(), ids);
+        ar.h_rast_cnt.clear();
+        ar.h_rast_ids.clear();
+        ar.h_rast_step0 = ar.t;
+        return NH_OK;
+    });
 }
 
 }  // extern "C"
