@@ -225,3 +225,64 @@ def test_paired_with_raster_drains_mid_run(window):
     rc, ri = O.OracleSim(net, O.default_cfg(**kw)).run_steps(301, record=True)
     assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
     assert rc.sum() > 300
+
+
+def random_net(seed):
+    """A random network: size, out-degrees, delays (a quarter of them 1),
+    weights, Izhikevich/AdEx mix all drawn from the seed."""
+    r = np.random.default_rng(seed)
+    n = int(r.integers(100, 7000))
+    deg = r.poisson(r.uniform(5, 60), n).astype(np.int64)
+    E = int(deg.sum())
+    src_ptr = np.zeros(n + 1, np.uint32)
+    src_ptr[1:] = np.cumsum(deg)
+    maxd = int(r.integers(1, 32))
+    delay = np.where(r.random(E) < 0.25, 1, r.integers(1, maxd + 1, E)).astype(np.uint16)
+    w = (r.uniform(0.0, 3.0, E) * np.where(r.random(E) < 0.2, -1.0, 1.0)).astype(np.float32)
+    adex = r.random() < 0.5
+    if adex:
+        model_ids = np.array([0, 1], np.uint8)
+        model = (r.random(n) < 0.5).astype(np.uint8)
+        P = 10
+        p = np.zeros((n, P), np.float32)
+        p[model == 0, :4] = [0.02, 0.2, -65.0, 8.0]
+        p[model == 1] = [281.0, 30.0, -70.6, -50.4, 2.0, 4.0, 144.0, 80.5, -70.6, 20.0]
+        x0 = np.zeros((n, 2), np.float32)
+        x0[model == 0] = [-65.0, -13.0]
+        x0[model == 1] = [-70.6, 0.0]
+    else:
+        model_ids = np.zeros(1, np.uint8)
+        model = np.zeros(n, np.uint8)
+        P = 4
+        p = np.tile(np.float32([0.02, 0.2, -65.0, 8.0]), (n, 1))
+        x0 = np.tile(np.float32([-65.0, -13.0]), (n, 1))
+    return dict(n=n, src_ptr=src_ptr, tgt=r.integers(0, n, E, dtype=np.uint32), w=w, delay=delay,
+                model_ids=model_ids, model=model, n_params=P, n_state=2,
+                p=p.reshape(-1), x0=x0.reshape(-1))
+
+
+@pytest.mark.parametrize("seed", [101, 202, 303, 404, 505, 606])
+def test_paired_random_networks_bit_exact(seed):
+    """Random networks (size, degrees, delays with many delay-1 synapses,
+    weights, model mix, noise, ring scale, step count, workgroup size):
+    the paired shape's raster and final state equal the oracle's."""
+    r = np.random.default_rng(seed + 1)
+    net = random_net(seed)
+    kw = dict(dt_ms=float(r.choice([0.1, 0.25, 0.5])), synapse_tau_ms=float(r.uniform(1, 10)),
+              noise_rate_hz=float(r.uniform(5, 60)), noise_kick_mv=20.0,
+              noise_sd=float(r.choice([0.0, 0.5, 1.0])), seed=int(r.integers(1, 1 << 30)),
+              int_weights=True, int_scale=float(r.choice([4096.0, 65536.0])))
+    steps = int(r.integers(50, 400))
+    block = int(r.choice([64, 128, 256, 512]))
+    print(f"case: random net seed {seed}: n={net['n']} E={len(net['tgt'])} "
+          f"d1={int((net['delay'] == 1).sum())} steps={steps} block={block} {kw}", flush=True)
+    sim = Sim(net, SimConfig(raster_steps=steps, step_shape=2, block=block, **kw))
+    assert sim.launch_shape() == 2
+    sim.run_steps(steps)
+    _, cnt, ids = sim.read_spikes()
+    ref = O.OracleSim(net, O.default_cfg(exp_mode=1, **kw))
+    rc, ri = ref.run_steps(steps, record=True)
+    assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
+    assert np.array_equal(sim.v(), ref.v().astype(np.float32))
+    assert np.array_equal(sim.state(1), ref.state(1).astype(np.float32))
+    assert np.array_equal(sim.syn_current(), ref.syn_current().astype(np.float32))
