@@ -338,6 +338,7 @@ typedef struct {
     uint64_t admitted, completed, peak_active;
     uint64_t cap_neurons, cap_synapses;
     double gpu_ms;
+    uint64_t paired_rounds;               /* rounds run with two steps per launch */
 } nh_server_stats;
 
 typedef struct nh_server nh_server;

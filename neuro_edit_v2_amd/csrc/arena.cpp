@@ -635,6 +635,7 @@ StepArgs Arena::args() const {
     a.watch = d_watch.as<uint32_t>();
     a.n_watch = n_watch;
     a.pl_ptr = d_pl_ptr.as<uint32_t>();
+    a.pl_hi = d_pl_hi.p ? d_pl_hi.as<uint32_t>() : (d_pl_ptr.p ? d_pl_ptr.as<uint32_t>() + 1 : nullptr);
     a.pl_ent = d_pl_ent.as<uint32_t>();
     a.pl_w = d_pl_w.p;
     a.pub = d_pub.as<uint64_t>();
@@ -719,6 +720,7 @@ int Arena::check_args(const StepArgs &a) const {
         TRY(need(a.xrec, "xrec"));
     }
     if (a.pl_ptr) {
+        TRY(need(a.pl_hi, "pl_hi"));
         TRY(need(a.pl_ent, "pl_ent"));
         TRY(need(a.pl_w, "pl_w"));
         TRY(need(a.pub, "pub"));
@@ -734,7 +736,7 @@ int Arena::check_args(const StepArgs &a) const {
 // most half the waves the device holds with this workgroup size -- and only
 // for the features it covers (no watched neurons, no custom-tau synapses).
 bool Arena::pair_usable() {
-    if (fused != 2 || !pair_tables || n_watch || n_x || dynamic || words == 0) return false;
+    if (fused != 2 || !pair_tables || n_watch || n_x || words == 0) return false;
     if (pair_cap_block != block) {
         pair_cap_waves = pair_resident_waves(ring_int, pmode, mmode, packed != 0, block);
         pair_cap_block = block;

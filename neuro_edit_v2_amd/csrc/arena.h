@@ -144,7 +144,7 @@ class Arena {
     // paired-step shape: delay-1 in-tables per wave, published spike words,
     // wait-abort flag; built when the shape may be used (the ring then has
     // one slot more than the deepest delay needs)
-    DBuf d_pl_ptr, d_pl_ent, d_pl_w, d_pub, d_pair_abort;
+    DBuf d_pl_ptr, d_pl_hi, d_pl_ent, d_pl_w, d_pub, d_pair_abort;  // d_pl_hi: served arena
     bool pair_tables = false;
     bool pair_launched = false;          // paired launches since the last abort check
     uint32_t pair_cap_block = 0, pair_cap_waves = 0;

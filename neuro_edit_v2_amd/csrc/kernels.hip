@@ -557,7 +557,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     const bool stim0 = a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
     const bool stim1 = a.stim_lo && a.stim_nz[(size_t)(a.k + 1) * a.stim_groups + net] != 0u;
     // this wave's delay-1 in-table; the first 64 entries are loaded here
-    const uint32_t pl_lo = a.pl_ptr[wave], pl_hi = a.pl_ptr[wave + 1];
+    const uint32_t pl_lo = a.pl_ptr[wave], pl_hi = a.pl_hi[wave];
     const RT *pl_w = (const RT *)a.pl_w;
     const bool has0 = pl_lo + lane < pl_hi;
     uint32_t ent0 = 0;

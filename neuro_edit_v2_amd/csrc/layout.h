@@ -125,11 +125,13 @@ struct StepArgs {
     const uint16_t *xs_d;
     const void *xs_w;
     // paired-step kernel (two steps per launch): per wave, the delay-1
-    // synapses into the wave ([pl_ptr[w], pl_ptr[w+1]) of pl_ent = source |
-    // target lane << 26, weight pl_w); the spike word each wave publishes
-    // for the first step (pub[2w + half] = tag << 32 | half word); the flag
-    // a wave raises when its bounded wait runs out
+    // synapses into the wave ([pl_ptr[w], pl_hi[w]) of pl_ent = source |
+    // target lane << 26, weight pl_w; pl_hi = pl_ptr + 1 for a packed
+    // table, the served arena keeps both); the spike word each wave
+    // publishes for the first step (pub[2w + half] = tag << 32 | half
+    // word); the flag a wave raises when its bounded wait runs out
     const uint32_t *pl_ptr;
+    const uint32_t *pl_hi;
     const uint32_t *pl_ent;
     const void *pl_w;
     uint64_t *pub;

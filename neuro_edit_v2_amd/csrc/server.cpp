@@ -104,9 +104,15 @@ struct Request {
     std::vector<float> stim_w;
     std::vector<float> stim;          // [steps][n_inputs]
     uint64_t stim_steps = 0;
+    // delay-1 in-table (two steps per launch), request-local: per wave the
+    // entry count, entries local source | target lane << 26, weights
+    std::vector<uint32_t> pl_cnt;
+    std::vector<uint32_t> pl_ent;
+    std::vector<uint32_t> pl_w;       // f32 or i32 bits, as the ring
+    uint64_t E1 = 0;
     // runtime
     uint32_t slot = 0;
-    uint64_t base = 0, sbase = 0, stbase = 0, t0 = 0;
+    uint64_t base = 0, sbase = 0, stbase = 0, plbase = 0, t0 = 0;
     uint64_t submit_ns = 0, admit_ns = 0, done_ns = 0, spikes = 0;
     bool done = false;
     int error = 0;
@@ -160,6 +166,8 @@ struct nh_server {
     std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
     std::vector<Request *> slot_req;
     FreeList fn, fs, fst;             // neurons (64-units), synapses, stimulus entries
+    FreeList fpl;                     // delay-1 in-table entries (paired shape)
+    bool pairs = false;               // two steps per launch where the grid allows
     Stage stage;                      // pinned staging of admission uploads
     std::vector<nh::PRow> table;
     nh_server_stats st{};
@@ -252,7 +260,10 @@ int build_arena(nh_server *s) {
     ar.n_nets = c.max_requests;
     ar.n_total = (uint32_t)c.cap_neurons;
     ar.words = ar.n_total / 64;
-    ar.D = c.max_delay + 1;
+    // two steps per launch needs the delay-1 in-tables (sources packed in
+    // 26 bits) and one ring slot more than the deepest delay
+    s->pairs = c.cap_neurons < (1ull << 26);
+    ar.D = c.max_delay + (s->pairs ? 2 : 1);
     ar.ring_int = c.int_weights ? 1 : 0;
     ar.pmode = 2;
     ar.mmode = 2;
@@ -310,6 +321,17 @@ int build_arena(nh_server *s) {
     TRY(A(ar.d_totals, 64));
     TRY(A(ar.d_rast_ctl, 32));
     if (c.n_windows) TRY(A(ar.d_wcnt, (uint64_t)c.n_windows * n * 4));
+    if (s->pairs) {
+        // per-wave table ranges start empty (free waves pull nothing)
+        TRY(A(ar.d_pl_ptr, (uint64_t)ar.words * 4));
+        TRY(A(ar.d_pl_hi, (uint64_t)ar.words * 4));
+        TRY(A(ar.d_pl_ent, c.cap_synapses * 4));
+        TRY(A(ar.d_pl_w, c.cap_synapses * 4));
+        TRY(A(ar.d_pub, (uint64_t)ar.words * 16));
+        TRY(A(ar.d_pair_abort, 4));
+        ar.pair_tables = true;
+        ar.fused = 2;
+    }
     HIPCHK(hipDeviceSynchronize());
     ar.use_graph = 1;
     ar.graph_steps_req = c.chunk_steps;
@@ -317,6 +339,7 @@ int build_arena(nh_server *s) {
     s->fn.init(n / 64);
     s->fs.init(c.cap_synapses);
     s->fst.init(c.cap_stim);
+    s->fpl.init(s->pairs ? c.cap_synapses : 0);
     s->slot_req.assign(c.max_requests, nullptr);
     s->st.cap_neurons = n;
     s->st.cap_synapses = c.cap_synapses;
@@ -350,8 +373,8 @@ int admit(nh_server *s, Request *r) {
     const uint64_t na = r->n_al;
     // (each piece rounds up to 256 B: 16 pieces of slack cover the fixed
     // ones, every new parameter row is a piece of its own)
-    const size_t need = 256 * 16 + r->rows.size() * 256 + na * (1 + 8 + 8 + 8) + (size_t)r->E * 8 +
-                        (size_t)r->n_stim * 8 + na / 64 * 2 + sizeof(nh::NetRec);
+    const size_t need = 256 * 20 + r->rows.size() * 256 + na * (1 + 8 + 8 + 8) + (size_t)r->E * 8 +
+                        (size_t)r->n_stim * 8 + na / 64 * (2 + 8) + (size_t)r->E1 * 8 + sizeof(nh::NetRec);
     HIPCHK(s->stage.reserve(need));
     Stage &sg = s->stage;
     // parameter rows into the shared table
@@ -418,6 +441,25 @@ int admit(nh_server *s, Request *r) {
     uint16_t *wn = sg.take<uint16_t>(na / 64);
     std::fill(wn, wn + na / 64, (uint16_t)r->slot);
     TRY(put(ar.d_wave_net, b / 64, wn, na / 64, st));
+    if (s->pairs) {
+        // delay-1 in-table: per-wave ranges in the pool, sources to arena ids
+        // (base + local < 2^26, so no carry into the lane field)
+        const uint64_t nw = na / 64;
+        uint32_t *plo = sg.take<uint32_t>(nw), *phi = sg.take<uint32_t>(nw);
+        uint64_t at = r->plbase;
+        for (uint64_t w = 0; w < nw; ++w) {
+            plo[w] = (uint32_t)at;
+            at += r->pl_cnt[w];
+            phi[w] = (uint32_t)at;
+        }
+        uint32_t *pe = sg.take<uint32_t>(r->E1), *pw = sg.take<uint32_t>(r->E1);
+        for (uint64_t e = 0; e < r->E1; ++e) pe[e] = r->pl_ent[e] + (uint32_t)b;
+        if (r->E1) std::memcpy(pw, r->pl_w.data(), r->E1 * 4);
+        TRY(put(ar.d_pl_ptr, b / 64, plo, nw, st));
+        TRY(put(ar.d_pl_hi, b / 64, phi, nw, st));
+        TRY(put(ar.d_pl_ent, r->plbase, pe, r->E1, st));
+        TRY(put(ar.d_pl_w, r->plbase, pw, r->E1, st));
+    }
     // ring columns, s, counters of the range start at zero
     nh::StepArgs a = ar.args();
     nh::launch_clear_range(a, (uint32_t)b, r->n_al, st);
@@ -475,6 +517,14 @@ bool try_admit(nh_server *s, Request *r) {
         s->fs.release(sb, r->E);
         return false;
     }
+    uint64_t plb = 0;
+    if (s->pairs && !s->fpl.alloc(r->E1, plb)) {
+        s->fn.release(nb, r->n_al / 64);
+        s->fs.release(sb, r->E);
+        s->fst.release(stb, r->n_stim);
+        return false;
+    }
+    r->plbase = plb;
     r->slot = slot;
     r->base = nb * 64;
     r->sbase = sb;
@@ -488,6 +538,7 @@ bool try_admit(nh_server *s, Request *r) {
         s->fn.release(nb, r->n_al / 64);
         s->fs.release(sb, r->E);
         s->fst.release(stb, r->n_stim);
+        if (s->pairs) s->fpl.release(plb, r->E1);
         s->cv_done.notify_all();
         return true;  // consumed (failed)
     }
@@ -515,6 +566,13 @@ int retire(nh_server *s, Request *r) {
     nh::NetRec &rec = ar.recs[r->slot];
     rec.active = 0;
     TRY(put(ar.d_nets, r->slot, &rec, 1, ar.stream));
+    std::vector<uint32_t> empty;
+    if (s->pairs) {
+        // the range's waves pull nothing from now on
+        empty.assign(r->n_al / 64, 0u);
+        TRY(put(ar.d_pl_ptr, r->base / 64, empty.data(), empty.size(), ar.stream));
+        TRY(put(ar.d_pl_hi, r->base / 64, empty.data(), empty.size(), ar.stream));
+    }
     HIPCHK(hipStreamSynchronize(ar.stream));
     uint64_t sp = 0, ev = 0;
     for (uint32_t i = 0; i < r->n; ++i) {
@@ -526,6 +584,7 @@ int retire(nh_server *s, Request *r) {
     s->fn.release(r->base / 64, r->n_al / 64);
     s->fs.release(r->sbase, r->E);
     s->fst.release(r->stbase, r->n_stim);
+    if (s->pairs) s->fpl.release(r->plbase, r->E1);
     s->slot_req[r->slot] = nullptr;
     s->active--;
     update_grid(s);
@@ -568,6 +627,7 @@ void worker_loop(nh_server *s) {
         }
         s->st.steps += k;
         s->st.gpu_ms += s->ar.last_run_ms;
+        if (s->ar.pair_usable()) s->st.paired_rounds++;
         for (uint32_t q = 0; q < s->cfg.max_requests; ++q) {
             Request *r = s->slot_req[q];
             if (r && s->ar.t - r->t0 >= r->total_steps) {
@@ -617,7 +677,9 @@ int nh_server_create(const nh_server_config *cfg, nh_server **out) {
         const double per_neuron = 8 + 4 + 4.0 * (c.max_delay + 1) + 1 + 8 + 8 + 8 + 4.0 * c.n_windows +
                                   (c.n_inputs ? 8 + 8 * 8 : 0);
         if (!c.cap_neurons) c.cap_neurons = std::min<uint64_t>((uint64_t)(budget * 0.5 / per_neuron), 16ull << 20);
-        if (!c.cap_synapses) c.cap_synapses = std::min<uint64_t>((uint64_t)(budget * 0.5 / 8.0), 1ull << 30);
+        // per synapse: target word + weight, and the same again for the
+        // delay-1 in-table pool of the paired shape
+        if (!c.cap_synapses) c.cap_synapses = std::min<uint64_t>((uint64_t)(budget * 0.5 / 16.0), 1ull << 30);
         c.cap_neurons = std::min<uint64_t>(c.cap_neurons, 1ull << 27) & ~uint64_t(63);
         if (c.cap_neurons < 64) return set_error(NH_ERR_CAPACITY, "not enough HBM for the arena");
         if (!c.cap_stim) c.cap_stim = c.n_inputs ? c.cap_neurons : 0;
@@ -687,6 +749,27 @@ int nh_server_submit(nh_server *s, const nh_net_desc *d, const nh_config *cfg, c
             nh::run_workers(work, nt);
             for (const char *e : err)
                 if (e) return set_error(std::strstr(e, "custom-tau") ? NH_ERR_UNSUPPORTED : NH_ERR_ARG, e);
+        }
+        if (s->pairs) {
+            // delay-1 in-table: rows are sorted by delay, so a row's delay-1
+            // synapses are its prefix; entries of a wave in source order
+            const uint32_t nw = r->n_al / 64;
+            r->pl_cnt.assign(nw, 0);
+            for (uint32_t i = 0; i < r->n; ++i)
+                for (uint32_t o = r->row_off[i]; o < r->row_off[i + 1] && (r->syn[o] >> 27) == 1u; ++o)
+                    r->pl_cnt[(r->syn[o] & 0x07FFFFFFu) / 64]++;
+            std::vector<uint32_t> at(nw + 1, 0);
+            for (uint32_t w = 0; w < nw; ++w) at[w + 1] = at[w] + r->pl_cnt[w];
+            r->E1 = at[nw];
+            r->pl_ent.resize(r->E1);
+            r->pl_w.resize(r->E1);
+            for (uint32_t i = 0; i < r->n; ++i)
+                for (uint32_t o = r->row_off[i]; o < r->row_off[i + 1] && (r->syn[o] >> 27) == 1u; ++o) {
+                    const uint32_t tg = r->syn[o] & 0x07FFFFFFu, e = at[tg / 64]++;
+                    r->pl_ent[e] = i | ((tg & 63u) << 26);
+                    std::memcpy(&r->pl_w[e],
+                                s->cfg.int_weights ? (const void *)&r->wq[o] : (const void *)&r->wf[o], 4);
+                }
         }
         // distinct parameter rows
         r->row_of.resize(r->n);

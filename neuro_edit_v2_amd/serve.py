@@ -30,7 +30,8 @@ class RequestStats(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("events", C.c_uint64), ("spikes", C.c_uint64),
                 ("admitted", C.c_uint64), ("completed", C.c_uint64), ("peak_active", C.c_uint64),
-                ("cap_neurons", C.c_uint64), ("cap_synapses", C.c_uint64), ("gpu_ms", C.c_double)]
+                ("cap_neurons", C.c_uint64), ("cap_synapses", C.c_uint64), ("gpu_ms", C.c_double),
+                ("paired_rounds", C.c_uint64)]
 
 
 def _sig():

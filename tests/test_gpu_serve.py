@@ -1,6 +1,7 @@
 """Served form: requests admitted while others run, batched into the same
-step launches; each request's result is the one it gets alone (int ring:
-bit-exact per-window spike counts)."""
+step launches (two steps per launch); each request's result is the one it
+gets alone with one step per launch (int ring: bit-exact per-window spike
+counts)."""
 import time
 
 import numpy as np
@@ -24,6 +25,7 @@ def request(seed, edge, ring=True):
 
 
 def alone(net, cfg, series):
+    cfg.step_shape = 1                    # one step per launch: the other shape
     s = Sim(net, cfg)
     for q in range(series.shape[1]):
         s.play_input(q, series[:, q])
@@ -44,6 +46,7 @@ def test_served_results_equal_alone_in_the_mix():
     st = srv.stats()
     assert st["completed"] == len(reqs)
     assert st["peak_active"] >= 2, "requests never overlapped on the device"
+    assert st["paired_rounds"] > 0, "the served arena never ran two steps per launch"
     t0s = {g[1]["t0"] for g in got}
     assert len(t0s) > 1, "every request started at the same step"
     for (net, cfg, ser), (counts, info) in zip(reqs, got):
