@@ -261,6 +261,8 @@ def main():
     ap.add_argument("--skip-served", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--skip-large", action="store_true")
+    ap.add_argument("--skip-other-ring", action="store_true",
+                    help="skip the run of the same network with the other ring")
     ap.add_argument("--large-n", type=int, default=1000000)
     args = ap.parse_args()
     if args.copy_probe or args.launch_probe:
@@ -379,6 +381,31 @@ def main():
                "wall_s": round(cwall, 3)}
 
     del sim
+    # the same network with the other ring (float if the headline ran int):
+    # the float-ring run of the 100k configuration beside the int-weights one
+    other = None
+    if not args.skip_other_ring:
+        kw2 = dict(kw, int_weights=not kw["int_weights"])
+        sim2 = Sim(net, SimConfig(exec_mode=args.exec_mode, step_shape=args.shape, **kw2))
+        sim2.run_steps(args.warmup)
+        sim2.synchronize()
+        e0 = sim2.events_total()
+        t1 = time.perf_counter()
+        sim2.run_steps(args.steps)
+        sim2.synchronize()
+        w2 = time.perf_counter() - t1
+        ev2 = sim2.events_total() - e0
+        spl2 = 2 if sim2.launch_shape() == 2 else 1
+        kb2 = spl2 * (args.n * B_NEURON + ev2 / args.steps * B_EVENT)
+        ka2 = sim2.last_run_ms() / (args.steps / spl2)
+        other = {"ring": "int Q16.16" if kw2["int_weights"] else "float",
+                 "value": round(ev2 / w2, 1), "unit": "synaptic events/s",
+                 "ms_per_step": round(w2 / args.steps * 1e3, 5),
+                 "sim_ms_per_s": round(args.steps * dt / w2, 1),
+                 "events": int(ev2),
+                 "roofline_frac": round(kb2 / (ka2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "avg_launch_us": round(ka2 * 1e3, 3), "steps_per_launch": spl2}
+        del sim2
     large = None if args.skip_large else bench_large(args, kw)
     batched = None
     if not args.skip_batched:
@@ -418,6 +445,7 @@ def main():
         "events_per_s_incl_build": round(events / (wall + t_build), 1),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "other_ring": other,
         "large": large,
         "batched": batched,
         "served": served,
