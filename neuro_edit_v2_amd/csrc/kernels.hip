@@ -19,8 +19,12 @@
 //    compacts the spikes of the step into a list; a second kernel scatters
 //    one spike per wave.  Kept as the reference shape and for comparison.
 //
+//  * k_pair ("paired" shape): two steps per launch where the whole grid is
+//    resident at once; the delay-1 arrivals of the first step are pulled by
+//    their targets' waves from published spike words (see its comment).
+//
 // The int ring (sim/src/kernels.rs:57-66) sums integers, which commute, so
-// both shapes give the CPU raster bit for bit whatever the atomic order.
+// every shape gives the CPU raster bit for bit whatever the atomic order.
 #include <hip/hip_runtime.h>
 #include "device.h"
 #include "layout.h"
