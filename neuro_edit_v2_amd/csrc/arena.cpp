@@ -738,7 +738,7 @@ int Arena::check_args(const StepArgs &a) const {
 bool Arena::pair_usable() {
     if (fused != 2 || !pair_tables || n_watch || n_x || words == 0) return false;
     if (pair_cap_block != block) {
-        pair_cap_waves = pair_resident_waves(ring_int, pmode, mmode, packed != 0, block);
+        pair_cap_waves = pair_resident_waves(ring_int, pmode, mmode, packed != 0, lean(), block);
         pair_cap_block = block;
     }
     return (uint64_t)neuron_blocks() * (block / 64) * 2 <= pair_cap_waves;
@@ -768,11 +768,11 @@ int Arena::launch_chunk(uint64_t t0, uint32_t k) {
         a.k = j;
         a.slot = (uint32_t)((t0 + j) % D);
         if (pair && j + 1 < k) {
-            launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, stream);
+            launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, lean(), stream);
             j += 2;
             continue;
         }
-        launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
+        launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, lean_step(), stream);
         if (!fused) launch_scatter(a, sb, ring_int, packed != 0, stream);
         if (n_watch) launch_watch(a, stream);
         ++j;
@@ -817,13 +817,13 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
             firsts.push_back(j);
             (void)hipEventRecord(ev[4 * j], stream);
             if (pair && j + 1 < k) {
-                launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, stream);
+                launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, lean(), stream);
                 (void)hipEventRecord(ev[4 * j + 1], stream);
                 pair_launched = true;
                 j += 2;
                 continue;
             }
-            launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, stream);
+            launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, lean_step(), stream);
             (void)hipEventRecord(ev[4 * j + 1], stream);
             if (!fused) {
                 (void)hipEventRecord(ev[4 * j + 2], stream);

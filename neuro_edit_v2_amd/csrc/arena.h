@@ -16,15 +16,15 @@ namespace nh {
 
 int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay);
 void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool push,
-                 bool packed, hipStream_t st);
+                 bool packed, bool lean, hipStream_t st);
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st);
 void launch_watch(const StepArgs &a, hipStream_t st);
 void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
 void launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t lane_bytes, hipStream_t st);
 void launch_empty(uint32_t blocks, hipStream_t st);
 void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
-                 hipStream_t st);
-uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, uint32_t block);
+                 bool lean, hipStream_t st);
+uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool lean, uint32_t block);
 
 int set_error(int code, const std::string &msg);
 
@@ -153,6 +153,13 @@ class Arena {
     uint32_t pair_spins = 0, pair_mute = 0;
     // true when the next chunk may use the paired kernel
     bool pair_usable();
+    // one network, no stimulus inputs, no custom-tau synapses, no window
+    // counters: the step kernels' lean variants (fewer arguments read)
+    bool lean() const { return n_nets == 1 && !dynamic && !d_stim_ptr.p && n_x == 0 && !d_wcnt.p; }
+    // the one-step kernel's lean variant pays off only on large grids, where
+    // the scalar-load throughput of many waves binds (measured: 0.17 us
+    // slower per step at 1.6k waves, 0.17 us faster at 4.7k, 0.54 us at 15.6k)
+    bool lean_step() const { return lean() && (uint64_t)neuron_blocks() * (block / 64) >= 3072; }
     int check_pair_abort();
     std::vector<uint32_t> h_deg_;      // total out-degree per arena neuron
     uint32_t n_win = 0;

@@ -266,8 +266,9 @@ __device__ __forceinline__ Noise noise_terms(const NetRec &nr, uint32_t local, u
 // (computed earlier by the caller); otherwise it is computed here, after
 // the parameter rows.  s holds the synaptic current before the step and
 // returns it decayed; xs is the model state.  Returns whether the neuron
-// spiked.
-template <int RING, int PMODE, int MMODE, bool PRE>
+// spiked.  LEAN: the arena has no custom-tau synapses (their pointers are
+// never read).
+template <int RING, int PMODE, int MMODE, bool PRE, bool LEAN>
 __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &nr, uint32_t i,
                                               uint32_t local, uint64_t tl, uint32_t slot,
                                               uint32_t kk, bool stim_on, uint32_t prow,
@@ -279,7 +280,7 @@ __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &n
     const float arr = Ring<RING>::get(rv, rsc);
     const float sv = s + arr;
     float I = sv + ext;
-    if (a.xt_ptr) {
+    if (!LEAN && a.xt_ptr) {
         // custom-tau traces into this neuron (cpu.rs:208-228): this
         // step's arrival, use, then decay -- each slot has one target,
         // so its thread owns it
@@ -356,11 +357,13 @@ __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &n
 // Per-neuron marks of a spike at step t (local step tl): lifetime count,
 // last spike step, window counter.  No-return atomics: nothing waits on a
 // read of the old count (each neuron owns its counter, so no contention).
+// LEAN: no window counters.
+template <bool LEAN>
 __device__ __forceinline__ void spike_marks(const StepArgs &a, const NetRec &nr, uint32_t i,
                                             uint64_t t, uint64_t tl) {
     atomicAdd(&a.pn_total[i], 1u);
     a.last_spike[i] = t < 0xFFFFFFFEull ? (uint32_t)t : 0xFFFFFFFEu;
-    if (a.wcnt && nr.win_steps) {
+    if (!LEAN && a.wcnt && nr.win_steps) {
         const uint64_t wi = tl / nr.win_steps;
         if (wi < a.n_win) atomicAdd(&a.wcnt[(size_t)wi * a.n_total + i], 1u);
     }
@@ -383,8 +386,12 @@ __device__ __forceinline__ void spike_words(const StepArgs &a, uint32_t wave, ui
 // PMODE: 0 parameter rows in the kernel arguments (<= 4 rows), 1 per-neuron
 // rows, 2 global row table.  MMODE: 0 izh, 1 adex, 2 mixed.
 // PUSH: the fused shape (own-spike push); otherwise the step appends its
-// spikes to the per-step list that k_scatter reads.
-template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED>
+// spikes to the per-step list that k_scatter reads.  LEAN: one network, no
+// stimulus inputs, no custom-tau synapses, no window counters -- the
+// pointers of those features are never read (every wave reads its
+// arguments with scalar loads, whose throughput a launch of many waves
+// feels).
+template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
 __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     using RT = typename Ring<RING>::T;
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -398,7 +405,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     // load at its point of use).
     NetRec nr;
     uint32_t net = 0;
-    if (a.single_net) {
+    if (LEAN || a.single_net) {
         nr = a.net0;
     } else {
         net = uniform((uint32_t)a.wave_net[wave]);
@@ -426,7 +433,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
     // does this network have a nonzero input this step?  (scalar load,
     // issued with the others)
-    const bool stim_on = a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
+    const bool stim_on = !LEAN && a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
 
     const uint32_t local = i - nr.base;
     const uint64_t tl = t - (uint64_t)nr.t0;
@@ -434,7 +441,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     bool spiked = false;
     float s_new = s_old;
     if (act)
-        spiked = neuron_update<RING, PMODE, MMODE, false>(a, nr, i, local, tl, slot, a.k, stim_on,
+        spiked = neuron_update<RING, PMODE, MMODE, false, LEAN>(a, nr, i, local, tl, slot, a.k, stim_on,
                                                           prow, rv, ext, Noise{false, 0.0f}, s_new,
                                                           xs);
 
@@ -446,7 +453,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
         // the stores would wait for the stores.  The push writes slots
         // t+1..t+D-1 only, never the slot cleared below.
         push_wave<RING, PACKED>(a, mask, row_lo, row_hi, slot, lane);
-        if (a.xs_lo) {
+        if (!LEAN && a.xs_lo) {
             // custom-tau rows (rare): one spike at a time
             for (uint64_t m = mask; m;) {
                 const uint32_t b = (uint32_t)__builtin_ctzll(m);
@@ -460,7 +467,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
         ring[ri] = (RT)0;
         a.x[i] = xs;
         a.s[i] = s_new;
-        if (spiked) spike_marks(a, nr, i, t, tl);
+        if (spiked) spike_marks<LEAN>(a, nr, i, t, tl);
     }
     if (lane == 0) spike_words(a, wave, t, rast_origin, mask);
     const uint32_t L = (uint32_t)(t % 3);
@@ -526,7 +533,7 @@ __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t en
     if (has && (((uint32_t)v >> (src & 31u)) & 1u)) atomicAdd(&acc_w[ent >> 26], w);
 }
 
-template <int RING, int PMODE, int MMODE, bool PACKED>
+template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
 __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     using RT = typename Ring<RING>::T;
     __shared__ RT acc[kMaxBlock];
@@ -538,7 +545,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     const uint32_t slot1 = slot0 + 1u == a.D ? 0u : slot0 + 1u;
     NetRec nr;
     uint32_t net = 0;
-    if (a.single_net) {
+    if (LEAN || a.single_net) {
         nr = a.net0;
     } else {
         net = uniform((uint32_t)a.wave_net[wave]);
@@ -558,8 +565,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
     const uint64_t t0 = *a.t_dev + a.k;
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
-    const bool stim0 = a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
-    const bool stim1 = a.stim_lo && a.stim_nz[(size_t)(a.k + 1) * a.stim_groups + net] != 0u;
+    const bool stim0 = !LEAN && a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
+    const bool stim1 = !LEAN && a.stim_lo && a.stim_nz[(size_t)(a.k + 1) * a.stim_groups + net] != 0u;
     // this wave's delay-1 in-table; the first 64 entries are loaded here
     const uint32_t pl_lo = a.pl_ptr[wave], pl_hi = a.pl_hi[wave];
     const RT *pl_w = (const RT *)a.pl_w;
@@ -577,7 +584,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     const bool act0 = local < nr.n && nr.active && (nr.t_len == 0 || tl0 < nr.t_len);
     bool sp0 = false;
     if (act0)
-        sp0 = neuron_update<RING, PMODE, MMODE, false>(a, nr, i, local, tl0, slot0, a.k, stim0, prow,
+        sp0 = neuron_update<RING, PMODE, MMODE, false, LEAN>(a, nr, i, local, tl0, slot0, a.k, stim0, prow,
                                                        rv0, ext, Noise{false, 0.0f}, s, xs);
     const uint64_t m0 = __ballot(sp0);
     const uint32_t tag = (uint32_t)t0 + 1u;   // never 0 before 2^32 steps: 0 is "unwritten"
@@ -591,7 +598,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     // after it is stored, and this work fills that time (done after the
     // poll it would add its own round trips to the end of the launch).
     if (m0) push_wave<RING, PACKED, true>(a, m0, row_lo, row_hi, slot0, lane);
-    if (sp0) spike_marks(a, nr, i, t0, tl0);
+    if (sp0) spike_marks<LEAN>(a, nr, i, t0, tl0);
     if (lane == 0) spike_words(a, wave, t0, rast_origin, m0);
 
     // step t1's noise does not depend on the pulled input: computed before
@@ -620,7 +627,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     // ---- step t1
     bool sp1 = false;
     if (act1)
-        sp1 = neuron_update<RING, PMODE, MMODE, true>(a, nr, i, local, tl1, slot1, a.k + 1u, stim1,
+        sp1 = neuron_update<RING, PMODE, MMODE, true, LEAN>(a, nr, i, local, tl1, slot1, a.k + 1u, stim1,
                                                       prow, rv1p, ext, nz1, s, xs);
     const uint64_t m1 = __ballot(sp1);
     if (m1) push_wave<RING, PACKED, false>(a, m1, row_lo, row_hi, slot1, lane);
@@ -632,7 +639,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
         a.x[i] = xs;
         a.s[i] = s;
     }
-    if (sp1) spike_marks(a, nr, i, t1, tl1);
+    if (sp1) spike_marks<LEAN>(a, nr, i, t1, tl1);
     if (lane == 0) spike_words(a, wave, t1, rast_origin, m1);
 }
 
@@ -712,55 +719,61 @@ void launch_empty(uint32_t blocks, hipStream_t st) {
 __global__ void k_advance(uint64_t *t_dev, uint32_t k) { *t_dev += k; }
 
 // ------------------------------------------------------------ dispatch
-template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED>
+template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
 static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED>), dim3(blocks), dim3(a.block), 0, st, a);
+    hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LEAN>), dim3(blocks), dim3(a.block), 0, st,
+                       a);
 }
 
 template <int RING, int PMODE, int MMODE>
-static void launch_m(const StepArgs &a, uint32_t blocks, bool push, bool packed, hipStream_t st) {
-    if (!push) go<RING, PMODE, MMODE, false, false>(a, blocks, st);
-    else if (packed) go<RING, PMODE, MMODE, true, true>(a, blocks, st);
-    else go<RING, PMODE, MMODE, true, false>(a, blocks, st);
+static void launch_m(const StepArgs &a, uint32_t blocks, bool push, bool packed, bool lean, hipStream_t st) {
+    if (!push) go<RING, PMODE, MMODE, false, false, false>(a, blocks, st);
+    else if (packed && lean) go<RING, PMODE, MMODE, true, true, true>(a, blocks, st);
+    else if (packed) go<RING, PMODE, MMODE, true, true, false>(a, blocks, st);
+    else if (lean) go<RING, PMODE, MMODE, true, false, true>(a, blocks, st);
+    else go<RING, PMODE, MMODE, true, false, false>(a, blocks, st);
 }
 
 template <int RING, int PMODE>
-static void launch_p(const StepArgs &a, uint32_t blocks, int mmode, bool push, bool packed, hipStream_t st) {
+static void launch_p(const StepArgs &a, uint32_t blocks, int mmode, bool push, bool packed, bool lean,
+                     hipStream_t st) {
     switch (mmode) {
-    case 0: launch_m<RING, PMODE, 0>(a, blocks, push, packed, st); break;
-    case 1: launch_m<RING, PMODE, 1>(a, blocks, push, packed, st); break;
-    default: launch_m<RING, PMODE, 2>(a, blocks, push, packed, st); break;
+    case 0: launch_m<RING, PMODE, 0>(a, blocks, push, packed, lean, st); break;
+    case 1: launch_m<RING, PMODE, 1>(a, blocks, push, packed, lean, st); break;
+    default: launch_m<RING, PMODE, 2>(a, blocks, push, packed, lean, st); break;
     }
 }
 
 template <int RING>
 static void launch_r(const StepArgs &a, uint32_t blocks, int pmode, int mmode, bool push, bool packed,
-                     hipStream_t st) {
+                     bool lean, hipStream_t st) {
     switch (pmode) {
-    case 0: launch_p<RING, 0>(a, blocks, mmode, push, packed, st); break;
-    case 1: launch_p<RING, 1>(a, blocks, mmode, push, packed, st); break;
-    default: launch_p<RING, 2>(a, blocks, mmode, push, packed, st); break;
+    case 0: launch_p<RING, 0>(a, blocks, mmode, push, packed, lean, st); break;
+    case 1: launch_p<RING, 1>(a, blocks, mmode, push, packed, lean, st); break;
+    default: launch_p<RING, 2>(a, blocks, mmode, push, packed, lean, st); break;
     }
 }
 
 void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool push,
-                 bool packed, hipStream_t st) {
-    if (ring == 0) launch_r<0>(a, blocks, pmode, mmode, push, packed, st);
-    else launch_r<1>(a, blocks, pmode, mmode, push, packed, st);
+                 bool packed, bool lean, hipStream_t st) {
+    if (ring == 0) launch_r<0>(a, blocks, pmode, mmode, push, packed, lean, st);
+    else launch_r<1>(a, blocks, pmode, mmode, push, packed, lean, st);
 }
 
 template <int RING, int PMODE, int MMODE>
-static const void *pair_fn(bool packed) {
-    return packed ? (const void *)k_pair<RING, PMODE, MMODE, true>
-                  : (const void *)k_pair<RING, PMODE, MMODE, false>;
+static const void *pair_fn(bool packed, bool lean) {
+    if (packed) return lean ? (const void *)k_pair<RING, PMODE, MMODE, true, true>
+                            : (const void *)k_pair<RING, PMODE, MMODE, true, false>;
+    return lean ? (const void *)k_pair<RING, PMODE, MMODE, false, true>
+                : (const void *)k_pair<RING, PMODE, MMODE, false, false>;
 }
 
-static const void *pair_kernel(int ring, int pmode, int mmode, bool packed) {
+static const void *pair_kernel(int ring, int pmode, int mmode, bool packed, bool lean) {
 #define NH_PAIR_M(R, P)                                                                           \
     switch (mmode) {                                                                              \
-    case 0: return pair_fn<R, P, 0>(packed);                                                      \
-    case 1: return pair_fn<R, P, 1>(packed);                                                      \
-    default: return pair_fn<R, P, 2>(packed);                                                     \
+    case 0: return pair_fn<R, P, 0>(packed, lean);                                                \
+    case 1: return pair_fn<R, P, 1>(packed, lean);                                                \
+    default: return pair_fn<R, P, 2>(packed, lean);                                               \
     }
 #define NH_PAIR_P(R)                                                                              \
     switch (pmode) {                                                                              \
@@ -775,19 +788,19 @@ static const void *pair_kernel(int ring, int pmode, int mmode, bool packed) {
 }
 
 void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
-                 hipStream_t st) {
+                 bool lean, hipStream_t st) {
     void *args[] = {(void *)&a};
-    (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed), dim3(blocks), dim3(a.block), args,
+    (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed, lean), dim3(blocks), dim3(a.block), args,
                           0, st);
 }
 
 // Waves of the paired-step kernel that the device holds at once with
 // workgroups of `block` threads (0 when the query fails).
-uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, uint32_t block) {
+uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool lean, uint32_t block) {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair_kernel(ring, pmode, mmode, packed),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair_kernel(ring, pmode, mmode, packed, lean),
                                                      (int)block, 0) != hipSuccess)
         return 0;
     return (uint32_t)cus * (uint32_t)per_cu * (block / 64u);
