@@ -259,6 +259,46 @@ __device__ __forceinline__ Noise noise_terms(const NetRec &nr, uint32_t local, u
     return z;
 }
 
+// Parameter row of neuron i (row index prow): kernel-argument rows
+// (PMODE 0), a global row table (2) or per-neuron rows (1).
+struct Params {
+    float p[kMaxParams];
+    uint32_t model;
+};
+template <int PMODE, int MMODE>
+__device__ __forceinline__ Params select_params(const StepArgs &a, uint32_t i, uint32_t prow) {
+    Params r_;
+    r_.model = (uint32_t)MMODE;
+    if (PMODE == 0) {
+#pragma unroll
+        for (int q = 0; q < kMaxParams; ++q) r_.p[q] = a.krow[0].p[q];
+        r_.model = a.krow[0].model;
+#pragma unroll
+        for (int r = 1; r < kKargRows; ++r) {
+            if ((uint32_t)r >= a.n_krows) break;  // uniform: only the rows in use
+            const bool sel = prow == (uint32_t)r;
+#pragma unroll
+            for (int q = 0; q < kMaxParams; ++q) r_.p[q] = sel ? a.krow[r].p[q] : r_.p[q];
+            r_.model = sel ? a.krow[r].model : r_.model;
+        }
+    } else if (PMODE == 2) {
+        const PRow &pt = a.ptab[prow];
+#pragma unroll
+        for (int q = 0; q < kMaxParams; ++q) r_.p[q] = pt.p[q];
+        r_.model = pt.model;
+    } else {
+        r_.model = MMODE == 2 ? a.mreg[i] : (uint32_t)MMODE;
+        if (MMODE == 0 && a.P == 4) {
+            const float4 v = *(const float4 *)(a.p + (size_t)i * 4);
+            r_.p[0] = v.x; r_.p[1] = v.y; r_.p[2] = v.z; r_.p[3] = v.w;
+        } else {
+            for (uint32_t q = 0; q < a.P && q < (uint32_t)kMaxParams; ++q)
+                r_.p[q] = a.p[(size_t)i * a.P + q];
+        }
+    }
+    return r_;
+}
+
 // One neuron's update at step t (local step tl), in the order of
 // sim/src/cpu.rs:217-260: ring arrival rv, custom-tau traces (ring slot
 // `slot`), stimulus (window row kk, skipped when stim_on is false), Poisson
@@ -267,13 +307,15 @@ __device__ __forceinline__ Noise noise_terms(const NetRec &nr, uint32_t local, u
 // the parameter rows.  s holds the synaptic current before the step and
 // returns it decayed; xs is the model state.  Returns whether the neuron
 // spiked.  LEAN: the arena has no custom-tau synapses (their pointers are
-// never read).
-template <int RING, int PMODE, int MMODE, bool PRE, bool LEAN>
+// never read).  HP: the parameter row comes in pp (selected once by the
+// caller); otherwise it is selected here.
+template <int RING, int PMODE, int MMODE, bool PRE, bool LEAN, bool HP>
 __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &nr, uint32_t i,
                                               uint32_t local, uint64_t tl, uint32_t slot,
                                               uint32_t kk, bool stim_on, uint32_t prow,
                                               typename Ring<RING>::T rv, float ext,
-                                              const Noise &nz, float &s, float2 &xs) {
+                                              const Noise &nz, const Params &pp, float &s,
+                                              float2 &xs) {
     using RT = typename Ring<RING>::T;
     bool spiked;
     const RingScale rsc(nr.int_scale);
@@ -304,35 +346,9 @@ __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &n
         for (uint32_t j = a.stim_lo[i]; j < a.stim_hi[i]; ++j)
             I += a.stim_w[j] * row[a.stim_input[j]];
     }
-    float pr[kMaxParams];
-    uint32_t model = (uint32_t)MMODE;
-    if (PMODE == 0) {
-#pragma unroll
-        for (int q = 0; q < kMaxParams; ++q) pr[q] = a.krow[0].p[q];
-        model = a.krow[0].model;
-#pragma unroll
-        for (int r = 1; r < kKargRows; ++r) {
-            if ((uint32_t)r >= a.n_krows) break;  // uniform: only the rows in use
-            const bool sel = prow == (uint32_t)r;
-#pragma unroll
-            for (int q = 0; q < kMaxParams; ++q) pr[q] = sel ? a.krow[r].p[q] : pr[q];
-            model = sel ? a.krow[r].model : model;
-        }
-    } else if (PMODE == 2) {
-        const PRow &r = a.ptab[prow];
-#pragma unroll
-        for (int q = 0; q < kMaxParams; ++q) pr[q] = r.p[q];
-        model = r.model;
-    } else {
-        model = MMODE == 2 ? a.mreg[i] : (uint32_t)MMODE;
-        if (MMODE == 0 && a.P == 4) {
-            const float4 v = *(const float4 *)(a.p + (size_t)i * 4);
-            pr[0] = v.x; pr[1] = v.y; pr[2] = v.z; pr[3] = v.w;
-        } else {
-            for (uint32_t q = 0; q < a.P && q < (uint32_t)kMaxParams; ++q)
-                pr[q] = a.p[(size_t)i * a.P + q];
-        }
-    }
+    const Params pp_ = HP ? pp : select_params<PMODE, MMODE>(a, i, prow);
+    const float *pr = pp_.p;
+    const uint32_t model = pp_.model;
     const bool is_adex = MMODE == 1 || (MMODE == 2 && model == 1);
     // sim/src/cpu.rs:240-245: kick * model gain (C for AdEx, adex.rs:189-191)
     const float gain = is_adex ? pr[0] : 1.0f;
@@ -441,9 +457,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     bool spiked = false;
     float s_new = s_old;
     if (act)
-        spiked = neuron_update<RING, PMODE, MMODE, false, LEAN>(a, nr, i, local, tl, slot, a.k, stim_on,
-                                                          prow, rv, ext, Noise{false, 0.0f}, s_new,
-                                                          xs);
+        spiked = neuron_update<RING, PMODE, MMODE, false, LEAN, false>(
+            a, nr, i, local, tl, slot, a.k, stim_on, prow, rv, ext, Noise{false, 0.0f}, Params{}, s_new, xs);
 
     // ---- wave64 ballot: the mask IS this wave's word of the spike bitmap
     const uint64_t mask = __ballot(spiked);
@@ -583,9 +598,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     const uint64_t tl0 = t0 - (uint64_t)nr.t0;
     const bool act0 = local < nr.n && nr.active && (nr.t_len == 0 || tl0 < nr.t_len);
     bool sp0 = false;
+    // the parameter row serves both steps
+    const Params pp = select_params<PMODE, MMODE>(a, i, prow);
     if (act0)
-        sp0 = neuron_update<RING, PMODE, MMODE, false, LEAN>(a, nr, i, local, tl0, slot0, a.k, stim0, prow,
-                                                       rv0, ext, Noise{false, 0.0f}, s, xs);
+        sp0 = neuron_update<RING, PMODE, MMODE, false, LEAN, true>(a, nr, i, local, tl0, slot0, a.k, stim0,
+                                                                  prow, rv0, ext, Noise{false, 0.0f}, pp,
+                                                                  s, xs);
     const uint64_t m0 = __ballot(sp0);
     const uint32_t tag = (uint32_t)t0 + 1u;   // never 0 before 2^32 steps: 0 is "unwritten"
     if (lane < 2u && !(a.pair_mute && wave == 0)) {
@@ -627,8 +645,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     // ---- step t1
     bool sp1 = false;
     if (act1)
-        sp1 = neuron_update<RING, PMODE, MMODE, true, LEAN>(a, nr, i, local, tl1, slot1, a.k + 1u, stim1,
-                                                      prow, rv1p, ext, nz1, s, xs);
+        sp1 = neuron_update<RING, PMODE, MMODE, true, LEAN, true>(a, nr, i, local, tl1, slot1, a.k + 1u,
+                                                                 stim1, prow, rv1p, ext, nz1, pp, s, xs);
     const uint64_t m1 = __ballot(sp1);
     if (m1) push_wave<RING, PACKED, false>(a, m1, row_lo, row_hi, slot1, lane);
 
