@@ -522,6 +522,19 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         TRY(upload(d_xs_d, xs_d));
         if (ring_int) TRY(upload(d_xs_w, xs_wq));
         else TRY(upload(d_xs_w, xs_wf));
+        xargs_h.n_x = n_x;
+        xargs_h.xt_ptr = d_xt_ptr.as<uint32_t>();
+        xargs_h.xt_idx = d_xt_idx.as<uint32_t>();
+        xargs_h.xdecay = d_xdecay.as<float>();
+        xargs_h.xsyn = d_xsyn.as<float>();
+        xargs_h.xring = d_xring.p;
+        xargs_h.xs_lo = d_xs_lo.as<uint32_t>();
+        xargs_h.xs_hi = d_xs_hi.as<uint32_t>();
+        xargs_h.xs_k = d_xs_k.as<uint32_t>();
+        xargs_h.xs_d = d_xs_d.as<uint16_t>();
+        xargs_h.xs_w = d_xs_w.p;
+        TRY(alloc(d_xargs, sizeof(XArgs)));
+        HIPCHK(hipMemcpy(d_xargs.p, &xargs_h, sizeof(XArgs), hipMemcpyHostToDevice));
     }
     // raster window: one spike word per wave per step (no atomics; the
     // host decodes ids, ascending, when it drains the window)
@@ -617,17 +630,7 @@ StepArgs Arena::args() const {
     }
     a.wcnt = d_wcnt.as<uint32_t>();
     a.n_win = n_win;
-    a.n_x = n_x;
-    a.xt_ptr = n_x ? d_xt_ptr.as<uint32_t>() : nullptr;
-    a.xt_idx = d_xt_idx.as<uint32_t>();
-    a.xdecay = d_xdecay.as<float>();
-    a.xsyn = d_xsyn.as<float>();
-    a.xring = d_xring.p;
-    a.xs_lo = n_x ? d_xs_lo.as<uint32_t>() : nullptr;
-    a.xs_hi = d_xs_hi.as<uint32_t>();
-    a.xs_k = d_xs_k.as<uint32_t>();
-    a.xs_d = d_xs_d.as<uint16_t>();
-    a.xs_w = d_xs_w.p;
+    a.xt = n_x ? d_xargs.as<XArgs>() : nullptr;
     a.syn = d_syn.as<uint32_t>();
     a.delay16 = d_delay16.as<uint16_t>();
     a.w = d_w.p;
@@ -703,17 +706,19 @@ int Arena::check_args(const StepArgs &a) const {
         TRY(need(a.stim_win, "stim_win"));
         TRY(need(a.stim_nz, "stim_nz"));
     }
-    if (a.n_x) {
-        TRY(need(a.xt_ptr, "xt_ptr"));
-        TRY(need(a.xt_idx, "xt_idx"));
-        TRY(need(a.xdecay, "xdecay"));
-        TRY(need(a.xsyn, "xsyn"));
-        TRY(need(a.xring, "xring"));
-        TRY(need(a.xs_lo, "xs_lo"));
-        TRY(need(a.xs_hi, "xs_hi"));
-        TRY(need(a.xs_k, "xs_k"));
-        TRY(need(a.xs_d, "xs_d"));
-        TRY(need(a.xs_w, "xs_w"));
+    if (a.xt) {
+        const XArgs &x = xargs_h;
+        if (x.n_x != n_x) return set_error(NH_ERR_ARG, "custom-tau arguments out of date");
+        TRY(need(x.xt_ptr, "xt_ptr"));
+        TRY(need(x.xt_idx, "xt_idx"));
+        TRY(need(x.xdecay, "xdecay"));
+        TRY(need(x.xsyn, "xsyn"));
+        TRY(need(x.xring, "xring"));
+        TRY(need(x.xs_lo, "xs_lo"));
+        TRY(need(x.xs_hi, "xs_hi"));
+        TRY(need(x.xs_k, "xs_k"));
+        TRY(need(x.xs_d, "xs_d"));
+        TRY(need(x.xs_w, "xs_w"));
     }
     if (a.n_watch) {
         TRY(need(a.watch, "watch"));

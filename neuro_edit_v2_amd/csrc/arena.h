@@ -140,6 +140,8 @@ class Arena {
     DBuf d_stim_hi;            // served arena: explicit stimulus range ends
     // custom-tau synapses
     DBuf d_xt_ptr, d_xt_idx, d_xdecay, d_xsyn, d_xring, d_xs_lo, d_xs_hi, d_xs_k, d_xs_d, d_xs_w;
+    DBuf d_xargs;              // one XArgs naming the buffers above
+    XArgs xargs_h{};           // its host copy
     uint32_t n_x = 0;
     // paired-step shape: delay-1 in-tables per wave, published spike words,
     // wait-abort flag; built when the shape may be used (the ring then has

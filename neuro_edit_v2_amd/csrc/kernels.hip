@@ -230,13 +230,14 @@ template <int RING>
 __device__ __forceinline__ void push_custom(const StepArgs &a, uint32_t src, uint32_t slot0,
                                             uint32_t lane) {
     using RT = typename Ring<RING>::T;
-    RT *xr = (RT *)a.xring;
-    const RT *w = (const RT *)a.xs_w;
-    const uint32_t lo = a.xs_lo[src], hi = a.xs_hi[src];
+    const XArgs &x = *a.xt;
+    RT *xr = (RT *)x.xring;
+    const RT *w = (const RT *)x.xs_w;
+    const uint32_t lo = x.xs_lo[src], hi = x.xs_hi[src];
     for (uint32_t e = lo + lane; e < hi; e += kWave) {
-        uint32_t sl = slot0 + a.xs_d[e];
+        uint32_t sl = slot0 + x.xs_d[e];
         sl = sl >= a.D ? sl - a.D : sl;
-        Ring<RING>::add(xr + (size_t)sl * a.n_x + a.xs_k[e], w[e]);
+        Ring<RING>::add(xr + (size_t)sl * x.n_x + x.xs_k[e], w[e]);
     }
 }
 
@@ -322,18 +323,19 @@ __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &n
     const float arr = Ring<RING>::get(rv, rsc);
     const float sv = s + arr;
     float I = sv + ext;
-    if (!LEAN && a.xt_ptr) {
+    if (!LEAN && a.xt) {
         // custom-tau traces into this neuron (cpu.rs:208-228): this
         // step's arrival, use, then decay -- each slot has one target,
         // so its thread owns it
-        RT *xr = (RT *)a.xring + (size_t)slot * a.n_x;
-        for (uint32_t j = a.xt_ptr[i]; j < a.xt_ptr[i + 1]; ++j) {
-            const uint32_t kx = a.xt_idx[j];
+        const XArgs &x = *a.xt;
+        RT *xr = (RT *)x.xring + (size_t)slot * x.n_x;
+        for (uint32_t j = x.xt_ptr[i]; j < x.xt_ptr[i + 1]; ++j) {
+            const uint32_t kx = x.xt_idx[j];
             const RT av = xr[kx];
             xr[kx] = (RT)0;
-            const float xv = a.xsyn[kx] + Ring<RING>::get(av, rsc);
+            const float xv = x.xsyn[kx] + Ring<RING>::get(av, rsc);
             I += xv;
-            a.xsyn[kx] = xv * a.xdecay[kx];
+            x.xsyn[kx] = xv * x.xdecay[kx];
         }
     }
     if (stim_on) {
@@ -468,7 +470,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
         // the stores would wait for the stores.  The push writes slots
         // t+1..t+D-1 only, never the slot cleared below.
         push_wave<RING, PACKED>(a, mask, row_lo, row_hi, slot, lane);
-        if (!LEAN && a.xs_lo) {
+        if (!LEAN && a.xt) {
             // custom-tau rows (rare): one spike at a time
             for (uint64_t m = mask; m;) {
                 const uint32_t b = (uint32_t)__builtin_ctzll(m);
@@ -675,7 +677,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_scatter(StepArgs a) {
         const uint32_t src = uniform(a.spk[(size_t)L * a.n_total + q]);
         const uint32_t lo = a.row_lo[src], hi = a.row_hi[src];
         push_row<RING, PACKED>(a, lo, hi, a.slot, lane);
-        if (a.xs_lo) push_custom<RING>(a, src, a.slot, lane);
+        if (a.xt) push_custom<RING>(a, src, a.slot, lane);
     }
 }
 

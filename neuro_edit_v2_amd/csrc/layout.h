@@ -48,6 +48,26 @@ static_assert(sizeof(PRow) == 48, "PRow size");
 
 constexpr int kKargRows = 4;        // parameter rows carried in the kernel arguments
 
+// Custom-tau synapses (own exponential trace, sim/src/state.rs:230-274): per
+// slot k a trace xsyn[k] with decay xdecay[k] and an arrival ring
+// xring[slot][k]; target-major lists xt for the fold-in, per-source rows xs
+// (k, delay, w) for the push.  Kept in device memory behind one pointer of
+// the step arguments: every wave reads its arguments with scalar loads, and
+// a group behind a pointer that may be null is read only where it is used.
+struct XArgs {
+    uint32_t n_x;
+    const uint32_t *xt_ptr;      // n_total+1
+    const uint32_t *xt_idx;
+    const float *xdecay;
+    float *xsyn;
+    void *xring;                 // D x n_x (float or int32)
+    const uint32_t *xs_lo;
+    const uint32_t *xs_hi;
+    const uint32_t *xs_k;
+    const uint16_t *xs_d;
+    const void *xs_w;
+};
+
 // Kernel arguments of one step.  Pointers may be null when a feature is off.
 struct StepArgs {
     uint32_t n_total;            // arena neurons (multiple of 64)
@@ -109,21 +129,7 @@ struct StepArgs {
     const uint16_t *delay16;     // unpacked delays (null when packed)
     const void *w;               // float or int32 per synapse
     uint64_t *totals;            // spare counters (cleared with the arena)
-    // custom-tau synapses (own exponential trace, sim/src/state.rs:230-274):
-    // per slot k a trace xsyn[k] with decay xdecay[k] and an arrival ring
-    // xring[slot][k]; target-major lists xt for the fold-in, per-source
-    // rows xs (k, delay, w) for the push
-    uint32_t n_x;
-    const uint32_t *xt_ptr;      // n_total+1 or null (no custom synapse)
-    const uint32_t *xt_idx;
-    const float *xdecay;
-    float *xsyn;
-    void *xring;                 // D x n_x (float or int32)
-    const uint32_t *xs_lo;
-    const uint32_t *xs_hi;
-    const uint32_t *xs_k;
-    const uint16_t *xs_d;
-    const void *xs_w;
+    const XArgs *xt;             // custom-tau synapses, or null (none)
     // paired-step kernel (two steps per launch): per wave, the delay-1
     // synapses into the wave ([pl_ptr[w], pl_hi[w]) of pl_ent = source |
     // target lane << 26, weight pl_w; pl_hi = pl_ptr + 1 for a packed
