@@ -123,7 +123,11 @@ def bench_batched(args, ring_int=True):
             "batched_wall_s": round(bwall, 5), "batched_gpu_ms": round(bgpu, 3),
             "plain_wall_s": round(plain_wall, 5),
             "batched_sim_ms_per_s": round(len(nets) * steps * dt / bwall, 1),
-            "events": int(bev), "events_plain": int(plain_ev), "spikes": spikes}
+            "events": int(bev), "events_plain": int(plain_ev), "spikes": spikes,
+            # whole-step roofline: the algorithmic bytes of a step (33 B per
+            # neuron, 17 B per event, as the headline) over the step's GPU time
+            "roofline_frac": round((n_tot * B_NEURON + bev / steps * B_EVENT)
+                                   / (bgpu / steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if ring_int:
         out["rasters_identical"] = bool(same and bev == plain_ev)
     else:

@@ -738,15 +738,22 @@ int Arena::check_args(const StepArgs &a) const {
 
 // The paired kernel waits on other waves of its own launch, so it runs only
 // when the whole grid fits on the device at once -- here: when it needs at
-// most half the waves the device holds with this workgroup size -- and only
-// for the features it covers (no watched neurons, no custom-tau synapses).
+// most half of the waves the device holds for that kernel and workgroup size
+// -- and only for the features it covers (no watched neurons, no custom-tau
+// synapses).  A single network whose grid is too large for k_pair may still
+// fit k_pair8, the build held to 8 waves per SIMD (pair_w8).
 bool Arena::pair_usable() {
+    pair_w8 = false;
     if (fused != 2 || !pair_tables || n_watch || n_x || words == 0) return false;
     if (pair_cap_block != block) {
-        pair_cap_waves = pair_resident_waves(ring_int, pmode, mmode, packed != 0, lean(), block);
+        pair_cap_waves = pair_resident_waves(ring_int, pmode, mmode, packed != 0, lean(), false, block);
+        pair_cap8_waves = lean() ? pair_resident_waves(ring_int, pmode, mmode, packed != 0, true, true, block) : 0;
         pair_cap_block = block;
     }
-    return (uint64_t)neuron_blocks() * (block / 64) * 2 <= pair_cap_waves;
+    const uint64_t need = (uint64_t)neuron_blocks() * (block / 64) * 2;
+    if (need <= pair_cap_waves) return true;
+    pair_w8 = lean() && need <= pair_cap8_waves;
+    return pair_w8;
 }
 
 // A paired launch whose bounded wait ran out leaves the state invalid:
@@ -773,7 +780,7 @@ int Arena::launch_chunk(uint64_t t0, uint32_t k) {
         a.k = j;
         a.slot = (uint32_t)((t0 + j) % D);
         if (pair && j + 1 < k) {
-            launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, lean(), stream);
+            launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, lean(), pair_w8, stream);
             j += 2;
             continue;
         }
@@ -822,7 +829,7 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
             firsts.push_back(j);
             (void)hipEventRecord(ev[4 * j], stream);
             if (pair && j + 1 < k) {
-                launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, lean(), stream);
+                launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, lean(), pair_w8, stream);
                 (void)hipEventRecord(ev[4 * j + 1], stream);
                 pair_launched = true;
                 j += 2;

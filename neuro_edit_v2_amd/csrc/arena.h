@@ -23,8 +23,9 @@ void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
 void launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t lane_bytes, hipStream_t st);
 void launch_empty(uint32_t blocks, hipStream_t st);
 void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
-                 bool lean, hipStream_t st);
-uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool lean, uint32_t block);
+                 bool lean, bool w8, hipStream_t st);
+uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool lean, bool w8,
+                             uint32_t block);
 
 int set_error(int code, const std::string &msg);
 
@@ -149,11 +150,13 @@ class Arena {
     DBuf d_pl_ptr, d_pl_hi, d_pl_ent, d_pl_w, d_pub, d_pair_abort;  // d_pl_hi: served arena
     bool pair_tables = false;
     bool pair_launched = false;          // paired launches since the last abort check
-    uint32_t pair_cap_block = 0, pair_cap_waves = 0;
+    uint32_t pair_cap_block = 0, pair_cap_waves = 0, pair_cap8_waves = 0;
+    bool pair_w8 = false;                // the next paired launches use the 8-wave build
     // test hook (failure handling): poll limit of the pair wait (0 = the
     // default), and whether wave 0 withholds its published word
     uint32_t pair_spins = 0, pair_mute = 0;
-    // true when the next chunk may use the paired kernel
+    // true when the next chunk may use the paired kernel (and which build:
+    // pair_w8)
     bool pair_usable();
     // one network, no stimulus inputs, no custom-tau synapses, no window
     // counters: the step kernels' lean variants (fewer arguments read)

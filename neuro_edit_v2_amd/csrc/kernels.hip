@@ -22,6 +22,8 @@
 //  * k_pair ("paired" shape): two steps per launch where the whole grid is
 //    resident at once; the delay-1 arrivals of the first step are pulled by
 //    their targets' waves from published spike words (see its comment).
+//    k_pair8 is the same code built for 8 waves per SIMD, for one network
+//    whose grid is too large for k_pair's 7.
 //
 // The int ring (sim/src/kernels.rs:57-66) sums integers, which commute, so
 // every shape gives the CPU raster bit for bit whatever the atomic order.
@@ -551,7 +553,7 @@ __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t en
 }
 
 template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
-__global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
+__device__ __forceinline__ void pair_steps(const StepArgs &a) {
     using RT = typename Ring<RING>::T;
     __shared__ RT acc[kMaxBlock];
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -661,6 +663,22 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
     }
     if (sp1) spike_marks<LEAN>(a, nr, i, t1, tl1);
     if (lane == 0) spike_words(a, wave, t1, rast_origin, m1);
+}
+
+template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
+__global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
+    pair_steps<RING, PMODE, MMODE, PACKED, LEAN>(a);
+}
+
+// The same kernel held to the register budget of 8 waves per SIMD (the
+// default build's 98 scalar registers allow 7): one network, used only where
+// k_pair's grid does not fit on the device at once but this one's does
+// (measured: 250k neurons 5.31 -> 3.99 us per step against one step per
+// launch; at 100k the spills it takes cost 0.14 us per step, so k_pair runs
+// there).
+template <int RING, int PMODE, int MMODE, bool PACKED>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pair8(StepArgs a) {
+    pair_steps<RING, PMODE, MMODE, PACKED, true>(a);
 }
 
 // ---------------------------------------------------------------- scatter
@@ -781,19 +799,22 @@ void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mm
 }
 
 template <int RING, int PMODE, int MMODE>
-static const void *pair_fn(bool packed, bool lean) {
+static const void *pair_fn(bool packed, bool lean, bool w8) {
+    if (w8) return packed ? (const void *)k_pair8<RING, PMODE, MMODE, true>
+                          : (const void *)k_pair8<RING, PMODE, MMODE, false>;
     if (packed) return lean ? (const void *)k_pair<RING, PMODE, MMODE, true, true>
                             : (const void *)k_pair<RING, PMODE, MMODE, true, false>;
     return lean ? (const void *)k_pair<RING, PMODE, MMODE, false, true>
                 : (const void *)k_pair<RING, PMODE, MMODE, false, false>;
 }
 
-static const void *pair_kernel(int ring, int pmode, int mmode, bool packed, bool lean) {
+// w8: the 8-waves-per-SIMD build (lean only)
+static const void *pair_kernel(int ring, int pmode, int mmode, bool packed, bool lean, bool w8) {
 #define NH_PAIR_M(R, P)                                                                           \
     switch (mmode) {                                                                              \
-    case 0: return pair_fn<R, P, 0>(packed, lean);                                                \
-    case 1: return pair_fn<R, P, 1>(packed, lean);                                                \
-    default: return pair_fn<R, P, 2>(packed, lean);                                               \
+    case 0: return pair_fn<R, P, 0>(packed, lean, w8);                                            \
+    case 1: return pair_fn<R, P, 1>(packed, lean, w8);                                            \
+    default: return pair_fn<R, P, 2>(packed, lean, w8);                                           \
     }
 #define NH_PAIR_P(R)                                                                              \
     switch (pmode) {                                                                              \
@@ -808,19 +829,21 @@ static const void *pair_kernel(int ring, int pmode, int mmode, bool packed, bool
 }
 
 void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
-                 bool lean, hipStream_t st) {
+                 bool lean, bool w8, hipStream_t st) {
     void *args[] = {(void *)&a};
-    (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed, lean), dim3(blocks), dim3(a.block), args,
-                          0, st);
+    (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed, lean, w8 && lean), dim3(blocks),
+                          dim3(a.block), args, 0, st);
 }
 
 // Waves of the paired-step kernel that the device holds at once with
 // workgroups of `block` threads (0 when the query fails).
-uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool lean, uint32_t block) {
+uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool lean, bool w8,
+                             uint32_t block) {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair_kernel(ring, pmode, mmode, packed, lean),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair_kernel(ring, pmode, mmode, packed, lean,
+                                                                          w8 && lean),
                                                      (int)block, 0) != hipSuccess)
         return 0;
     return (uint32_t)cus * (uint32_t)per_cu * (block / 64u);

@@ -1,7 +1,8 @@
 """GPU parity at the benchmark sizes: the 100k-neuron headline network
 (int ring at both scales, bit-exact raster over the whole 1 s run; float
 ring within the stated tolerances), the 1M-neuron network (int ring,
-bit-exact), and the 256-network batched workload (identical rasters packed
+bit-exact), a 250k network in paired launches, and the 256-network batched
+workload (identical rasters packed
 and alone, a sample of networks against the oracle)."""
 import numpy as np
 import pytest
@@ -102,6 +103,29 @@ def test_million_neuron_int_raster_bit_exact():
     sim.close()
     ref = O.OracleSim(net, O.default_cfg(**kw))
     del net
+    rc, ri = ref.run_steps(steps, record=True)
+    print(f"spikes gpu={cnt.sum()} oracle={rc.sum()}")
+    assert rc.sum() > 20000
+    assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
+    assert np.array_equal(v, ref.v().astype(np.float32))
+    assert np.array_equal(u, ref.state(1).astype(np.float32))
+
+
+def test_paired_250k_eight_wave_build_bit_exact():
+    """250k neurons (3,907 waves): too many for the paired kernel's default
+    build at 7 waves per SIMD with the co-residency margin, so the run takes
+    its 8-waves-per-SIMD build; 200 steps bit-exact against the oracle."""
+    steps = 200
+    net = N.cube_net(250000, mean_deg=100.0, seed=2, w_scale=0.7)
+    kw = dict(HEAD, int_weights=True, int_scale=65536.0)
+    print("case: 250k paired (8-wave build), 200 steps", flush=True)
+    sim = Sim(net, SimConfig(raster_steps=steps, **kw))
+    assert sim.launch_shape() == 2
+    sim.run_steps(steps)
+    _, cnt, ids = sim.read_spikes()
+    v, u = sim.v(), sim.state(1)
+    sim.close()
+    ref = O.OracleSim(net, O.default_cfg(**kw))
     rc, ri = ref.run_steps(steps, record=True)
     print(f"spikes gpu={cnt.sum()} oracle={rc.sum()}")
     assert rc.sum() > 20000
