@@ -279,10 +279,22 @@ __device__ __forceinline__ Params select_params(const StepArgs &a, uint32_t i, u
 #pragma unroll
         for (int r = 1; r < kKargRows; ++r) {
             if ((uint32_t)r >= a.n_krows) break;  // uniform: only the rows in use
-            const bool sel = prow == (uint32_t)r;
+            // One v_bfi_b32 per parameter, (m & row) | (~m & current), with
+            // the row value read from its scalar register.  A select would
+            // take a move and a v_cndmask per parameter (the lane mask already
+            // uses the instruction's one scalar operand), and the compiler
+            // turns the plain-C bit select back into that pair.
+            const uint32_t m = prow == (uint32_t)r ? 0xFFFFFFFFu : 0u;
 #pragma unroll
-            for (int q = 0; q < kMaxParams; ++q) r_.p[q] = sel ? a.krow[r].p[q] : r_.p[q];
-            r_.model = sel ? a.krow[r].model : r_.model;
+            for (int q = 0; q < kMaxParams; ++q) {
+                uint32_t o;
+                asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(o) : "v"(m), "s"(__float_as_uint(a.krow[r].p[q])),
+                    "v"(__float_as_uint(r_.p[q])));
+                r_.p[q] = __uint_as_float(o);
+            }
+            uint32_t om;
+            asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(om) : "v"(m), "s"(a.krow[r].model), "v"(r_.model));
+            r_.model = om;
         }
     } else if (PMODE == 2) {
         const PRow &pt = a.ptab[prow];
