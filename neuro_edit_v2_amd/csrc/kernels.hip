@@ -423,11 +423,26 @@ __device__ __forceinline__ void spike_words(const StepArgs &a, uint32_t wave, ui
 // pointers of those features are never read (every wave reads its
 // arguments with scalar loads, whose throughput a launch of many waves
 // feels).
+// Packed arenas of several networks: a launch whose grid is a multiple of
+// kBands workgroups visits the arena in kBands interleaved bands (workgroup
+// b takes block (b % kBands) * (grid / kBands) + b / kBands), so that the
+// workgroups in flight on a SIMD come from networks far apart in the arena
+// -- a mix of models (an AdEx wave is VALU-heavy, an Izhikevich wave is
+// memory-bound) instead of one network's waves side by side.  33 is coprime
+// to the 8 XCDs the workgroups are dealt to, so every XCD sees every band
+// (measured on the 256-network workload, us per step: 24.6 in arena order;
+// 9 bands 24.2, 17 23.5, 33 23.3, 65 24.1).  Any grid size gives the same
+// results: the map is a permutation of the grid's workgroups.
+constexpr uint32_t kBands = 33;
+
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
 __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     using RT = typename Ring<RING>::T;
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
+    uint32_t blk = blockIdx.x;
+    if (!LEAN && PUSH && !a.single_net && gridDim.x % kBands == 0u)
+        blk = (blk % kBands) * (gridDim.x / kBands) + blk / kBands;
+    const uint32_t wave = blk * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
     if (wave >= a.words) return;
     const uint32_t i = wave * kWave + lane;
     const uint32_t slot = a.slot;
@@ -771,6 +786,9 @@ __global__ void k_advance(uint64_t *t_dev, uint32_t k) { *t_dev += k; }
 // ------------------------------------------------------------ dispatch
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
 static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
+    // a packed arena's grid is padded to whole bands (waves past the arena
+    // return at once)
+    if (!LEAN && PUSH && !a.single_net && blocks >= 4u * kBands) blocks = (blocks + kBands - 1u) / kBands * kBands;
     hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LEAN>), dim3(blocks), dim3(a.block), 0, st,
                        a);
 }
