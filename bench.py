@@ -195,12 +195,12 @@ def bench_served(args, ring_int=True):
 def bench_large(args, kw):
     """The 1M-neuron network (random cube, Poisson(100) out-degree, ~100M
     synapses), same generator, noise and ring as the headline workload,
-    400 steps after 400 warm-up steps: the step kernel at a size where the
+    400 steps after 2000 warm-up steps: the step kernel at a size where the
     chip is full, with its roofline fraction."""
     from neuro_edit_v2_amd import Sim, SimConfig
     from neuro_edit_v2_amd import net as N
 
-    n, steps, warm, dt = args.large_n, 400, 400, kw["dt_ms"]
+    n, steps, warm, dt = args.large_n, 400, 2000, kw["dt_ms"]
     t_gen = time.time()
     net = N.cube_net(n, mean_deg=100.0, seed=1, w_scale=0.7)
     t_gen = time.time() - t_gen
