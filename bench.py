@@ -92,7 +92,7 @@ def bench_batched(args, ring_int=True):
     bev = b.events_total() - bev0
     plain_wall, plain_ev, same, spikes = 0.0, 0, True, 0
     dev = []
-    for k, net in enumerate(nets):
+    for k, net in enumerate(nets if not args.batch_no_plain else []):
         s = Sim(net, cfgs[k])
         for q in range(8):
             s.play_input(q, series[k][:, q])
@@ -118,8 +118,8 @@ def bench_batched(args, ring_int=True):
             "autotune": tuned,
             "batched_us_per_step": round(bgpu / steps * 1e3, 3),
             "batched_events_per_s": round(bev / bwall, 1),
-            "plain_sequential_events_per_s": round(plain_ev / plain_wall, 1),
-            "speedup": round((bev / bwall) / (plain_ev / plain_wall), 2),
+            "plain_sequential_events_per_s": round(plain_ev / plain_wall, 1) if plain_wall else None,
+            "speedup": round((bev / bwall) / (plain_ev / plain_wall), 2) if plain_wall else None,
             "batched_wall_s": round(bwall, 5), "batched_gpu_ms": round(bgpu, 3),
             "plain_wall_s": round(plain_wall, 5),
             "batched_sim_ms_per_s": round(len(nets) * steps * dt / bwall, 1),
@@ -128,7 +128,9 @@ def bench_batched(args, ring_int=True):
             # neuron, 17 B per event, as the headline) over the step's GPU time
             "roofline_frac": round((n_tot * B_NEURON + bev / steps * B_EVENT)
                                    / (bgpu / steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    if ring_int:
+    if args.batch_no_plain:
+        pass
+    elif ring_int:
         out["rasters_identical"] = bool(same and bev == plain_ev)
     else:
         # float sums in atomic order: per-network spike totals, batched vs plain
@@ -265,6 +267,8 @@ def main():
     ap.add_argument("--batch-nets", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched form: keep the default launch shape")
+    ap.add_argument("--batch-no-plain", action="store_true",
+                    help="batched form: skip the one-at-a-time runs (for profiling the packed kernel)")
     ap.add_argument("--skip-served", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--skip-large", action="store_true")

@@ -269,9 +269,12 @@ struct Params {
     uint32_t model;
 };
 // BFI: rows selected with one v_bfi_b32 per parameter (see below); otherwise
-// with a select per parameter, which the compiler schedules more freely --
-// measured faster in the lean one-step kernel (1M neurons: 10.18 against
-// 10.46 us per step), slower in the paired and multi-network kernels.
+// with a select per parameter, which the compiler schedules more freely.
+// Measured: the bit inserts are faster in the paired kernel (100k: 2.725
+// against 2.745 us per step) and in the one-step kernel of the mixed
+// Izhikevich/AdEx batch (24.62 against 25.02 us), slower in the
+// Izhikevich-only one-step kernel (1M: 10.46 against 10.18 us; 100k one step
+// per launch: 3.69 against 3.56 us).
 template <int PMODE, int MMODE, bool BFI>
 __device__ __forceinline__ Params select_params(const StepArgs &a, uint32_t i, uint32_t prow) {
     Params r_;
@@ -373,7 +376,7 @@ __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &n
         for (uint32_t j = a.stim_lo[i]; j < a.stim_hi[i]; ++j)
             I += a.stim_w[j] * row[a.stim_input[j]];
     }
-    const Params pp_ = HP ? pp : select_params<PMODE, MMODE, !LEAN>(a, i, prow);
+    const Params pp_ = HP ? pp : select_params<PMODE, MMODE, MMODE != 0>(a, i, prow);
     const float *pr = pp_.p;
     const uint32_t model = pp_.model;
     const bool is_adex = MMODE == 1 || (MMODE == 2 && model == 1);
