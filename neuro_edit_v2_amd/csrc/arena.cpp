@@ -96,7 +96,20 @@ int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay) {
     if (d.n > 0x7FFFFFC0u) return set_error(NH_ERR_CAPACITY, "network exceeds 2^31 neurons");
     for (uint32_t m = 0; m < d.n_models; ++m)
         if (d.model_ids[m] > 1) return set_error(NH_ERR_ARG, "unknown model id " + std::to_string(d.model_ids[m]));
+    // Per-neuron checks as reductions first (vectorised): the models in
+    // use, whether any parameter or initial value has the all-ones exponent
+    // of an infinity or NaN, whether src_ptr ever decreases; the loops that
+    // name the offence run only when a reduction fails.
+    uint32_t mmax = 0, used = 0;
     for (uint32_t i = 0; i < d.n; ++i) {
+        mmax = std::max<uint32_t>(mmax, d.model[i]);
+        used |= 1u << (d.model[i] & 31u);
+    }
+    bool bad_model = d.n && mmax >= d.n_models;
+    for (uint32_t m = 0; !bad_model && m < d.n_models; ++m)
+        if ((used >> m) & 1u)
+            bad_model = (d.model_ids[m] == 1 && d.n_params < 10) || (d.model_ids[m] == 0 && d.n_params < 4);
+    for (uint32_t i = 0; bad_model && i < d.n; ++i) {
         if (d.model[i] >= d.n_models)
             return set_error(NH_ERR_ARG, "neuron " + std::to_string(i) + " names model " +
                                              std::to_string(d.model[i]) + " outside the model table");
@@ -104,18 +117,50 @@ int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay) {
         if (id == 1 && d.n_params < 10) return set_error(NH_ERR_ARG, "adex rows need 10 params");
         if (id == 0 && d.n_params < 4) return set_error(NH_ERR_ARG, "izh rows need 4 params");
     }
-    for (uint64_t q = 0; q < (uint64_t)d.n * d.n_params; ++q)
-        if (!fin(d.p[q])) return set_error(NH_ERR_ARG, "non-finite parameter");
-    for (uint64_t q = 0; q < (uint64_t)d.n * 2; ++q)
-        if (!fin(d.x0[q])) return set_error(NH_ERR_ARG, "non-finite initial state");
+    auto all_finite = [](const float *v, uint64_t n) {
+        uint32_t nonfin = 0;
+        for (uint64_t q = 0; q < n; ++q) {
+            uint32_t b;
+            std::memcpy(&b, &v[q], 4);
+            nonfin |= (b & 0x7F800000u) == 0x7F800000u;
+        }
+        return nonfin == 0;
+    };
+    if (!all_finite(d.p, (uint64_t)d.n * d.n_params)) return set_error(NH_ERR_ARG, "non-finite parameter");
+    if (!all_finite(d.x0, (uint64_t)d.n * 2)) return set_error(NH_ERR_ARG, "non-finite initial state");
     if (d.n && d.src_ptr[0] != 0) return set_error(NH_ERR_ARG, "src_ptr[0] must be 0");
-    for (uint32_t i = 0; i < d.n; ++i)
-        if (d.src_ptr[i + 1] < d.src_ptr[i]) return set_error(NH_ERR_ARG, "src_ptr not monotone");
+    uint32_t down = 0;
+    for (uint32_t i = 0; i < d.n; ++i) down |= d.src_ptr[i + 1] < d.src_ptr[i];
+    if (down) return set_error(NH_ERR_ARG, "src_ptr not monotone");
     const uint64_t E = d.n ? d.src_ptr[d.n] : 0;
     if (E != d.n_syn) return set_error(NH_ERR_ARG, "src_ptr[n] != n_syn");
     uint32_t maxd = 1;
     const double lim = 2147483647.0;
-    for (uint64_t j = 0; j < E; ++j) {
+    // Fast pass: the synapse checks as reductions the compiler vectorises --
+    // largest target, smallest and largest delay, largest |w| taken on its
+    // bit pattern (monotone for non-negative floats; above the infinity
+    // pattern only NaN) -- and the per-synapse pass below, which names the
+    // first offending synapse, only when one of them fails.  The int-ring
+    // bound is monotone in |w|, so its largest value decides it.
+    bool need_slow = d.tau != nullptr;
+    if (!need_slow && E) {
+        uint32_t tmax = 0, wbits = 0;
+        uint16_t dmin = 0xFFFF, dmax = 0;
+        for (uint64_t j = 0; j < E; ++j) {
+            tmax = std::max(tmax, d.tgt[j]);
+            dmin = std::min(dmin, d.delay[j]);
+            dmax = std::max(dmax, d.delay[j]);
+            uint32_t b;
+            std::memcpy(&b, &d.w[j], 4);
+            wbits = std::max(wbits, b & 0x7FFFFFFFu);
+        }
+        float wmax;
+        std::memcpy(&wmax, &wbits, 4);
+        need_slow = tmax >= d.n || dmin < 1 || wbits >= 0x7F800000u ||
+                    (c.int_weights && std::fabs((double)wmax * c.int_scale) >= lim);
+        maxd = std::max<uint32_t>(maxd, dmax);
+    }
+    for (uint64_t j = 0; need_slow && j < E; ++j) {
         if (d.tgt[j] >= d.n) return set_error(NH_ERR_ARG, "synapse target out of range");
         if (d.delay[j] < 1) return set_error(NH_ERR_ARG, "delay must be >= 1 step");
         if (!fin(d.w[j])) return set_error(NH_ERR_ARG, "non-finite synapse weight");

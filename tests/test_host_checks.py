@@ -21,10 +21,14 @@ def check(net, **cfg):
     return rc, L.lib().nh_last_error().decode()
 
 
-@pytest.fixture()
-def net():
+@pytest.fixture(params=[True, False], ids=["tau", "no_tau"])
+def net(request):
+    """With a per-synapse tau array the synapse checks run one synapse at a
+    time; without it they run as reductions first (and one synapse at a
+    time only to name a failure), so every defect is planted in both."""
     n = N.grid_net(5, n_inputs=2, per_input=4, seed=3)
-    n["tau"] = np.zeros(len(n["tgt"]), np.float32)
+    if request.param:
+        n["tau"] = np.zeros(len(n["tgt"]), np.float32)
     return n
 
 
@@ -57,9 +61,18 @@ def _mut(net, key, fn):
     (lambda n: n, dict(int_weights=True, int_scale=0.0), "int_scale must be positive"),
     (lambda n: _mut(n, "w", lambda a: a.__setitem__(0, 1e6)), dict(int_weights=True, int_scale=65536.0),
      "exceeds the int32 ring"),
+    (lambda n: _mut(n, "w", lambda a: a.__setitem__(len(a) - 1, -np.inf)), {}, "non-finite synapse weight"),
+    (lambda n: _mut(n, "w", lambda a: a.__setitem__(7, -np.float32(np.nan))), {}, "non-finite synapse weight"),
+    (lambda n: _mut(n, "tgt", lambda a: a.__setitem__(len(a) - 1, 0xFFFFFFFF)), {}, "target out of range"),
+    (lambda n: _mut(n, "x0", lambda a: a.__setitem__(len(a) - 1, -np.inf)), {}, "non-finite initial state"),
+    (lambda n: dict(_mut(n, "model_ids", lambda a: a.__setitem__(0, 1)), n_params=4), {},
+     "adex rows need 10 params"),
 ], ids=["tgt", "delay0", "w_nan", "tau_neg", "src_ptr", "model", "model_id", "param_inf", "x0_nan",
-        "stim_input", "dt0", "dt_big", "sd_neg", "scale0", "int_overflow"])
+        "stim_input", "dt0", "dt_big", "sd_neg", "scale0", "int_overflow", "w_neg_inf", "w_neg_nan",
+        "tgt_last", "x0_last_inf", "adex_short_rows"])
 def test_planted_defect_is_refused(net, defect, cfg, reason):
+    if reason.startswith("tau") and net.get("tau") is None:
+        pytest.skip("no tau array to plant the defect in")
     rc, msg = check(defect(net), **cfg)
     assert rc == L.NH_ERR_ARG, msg
     assert reason in msg, msg
