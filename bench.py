@@ -210,15 +210,14 @@ def bench_large(args, kw):
     sim = Sim(net, SimConfig(exec_mode=args.exec_mode, step_shape=args.shape, **kw))
     del net
     sim.run_steps(warm)
+    sim.mark_counters()
     sim.synchronize()
-    ev0, sp0 = sim.events_total(), sim.spikes_total()
     t0 = time.perf_counter()
     sim.run_steps(steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = sim.last_run_ms()
-    events = sim.events_total() - ev0
-    spikes = sim.spikes_total() - sp0
+    spikes, events = sim.counts_since_mark()
     ev_step = events / steps
     spl = 2 if sim.launch_shape() == 2 else 1     # steps per launch
     kbytes = spl * (n * B_NEURON + ev_step * B_EVENT)
@@ -312,16 +311,19 @@ def main():
     sim = Sim(net, SimConfig(exec_mode=args.exec_mode, step_shape=args.shape, **kw))
     sim.synchronize()
     t_build = time.perf_counter() - t_build      # host CSR -> HBM upload and set-up
+    # The window-start counters are a device-side snapshot enqueued behind
+    # the warm-up: reading them on the host here would leave the device idle
+    # for a host round trip right before the timed steps (at 1M neurons a
+    # 1.2 ms read made the next 400 steps 4-5 % slower: the clocks drop).
     sim.run_steps(args.warmup)
+    sim.mark_counters()
     sim.synchronize()
-    ev0, sp0 = sim.events_total(), sim.spikes_total()
     t0 = time.perf_counter()
     sim.run_steps(args.steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = sim.last_run_ms()
-    events = sim.events_total() - ev0
-    spikes = sim.spikes_total() - sp0
+    spikes, events = sim.counts_since_mark()
     value = events / wall
 
     # Dominant-kernel roofline.  Fused shape: the step kernel is the only
@@ -399,13 +401,13 @@ def main():
         kw2 = dict(kw, int_weights=not kw["int_weights"])
         sim2 = Sim(net, SimConfig(exec_mode=args.exec_mode, step_shape=args.shape, **kw2))
         sim2.run_steps(args.warmup)
+        sim2.mark_counters()
         sim2.synchronize()
-        e0 = sim2.events_total()
         t1 = time.perf_counter()
         sim2.run_steps(args.steps)
         sim2.synchronize()
         w2 = time.perf_counter() - t1
-        ev2 = sim2.events_total() - e0
+        ev2 = sim2.counts_since_mark()[1]
         spl2 = 2 if sim2.launch_shape() == 2 else 1
         kb2 = spl2 * (args.n * B_NEURON + ev2 / args.steps * B_EVENT)
         ka2 = sim2.last_run_ms() / (args.steps / spl2)

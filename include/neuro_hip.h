@@ -137,6 +137,14 @@ int nh_mean_rate_hz(nh_sim *s, double *out);
 int nh_read_pn_total(nh_sim *s, uint64_t *out, uint32_t n);
 /* Synaptic events delivered so far (one per spike x out-synapse). */
 int nh_events_total(nh_sim *s, uint64_t *out);
+/* Measurement helpers (additions, no counterpart in the reference): a
+ * snapshot of the per-neuron spike totals taken on the device in stream
+ * order -- the host does not wait, so a timed run enqueued right after it
+ * starts on a busy device -- and the spikes and synaptic events since the
+ * last snapshot (synchronises).  A reset clears the snapshot
+ * (NH_ERR_ARG until the next mark). */
+int nh_mark_counters(nh_sim *s);
+int nh_counts_since_mark(nh_sim *s, uint64_t *spikes, uint64_t *events);
 /* Backend::spike_row(back) -- lib.rs:148; NH_ERR_NONE for None. */
 int nh_spike_row(nh_sim *s, uint64_t back, uint64_t *words, uint32_t n_words);
 /* Backend::last_spikes -- lib.rs:142 (ascending ids) */

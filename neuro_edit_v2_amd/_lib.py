@@ -92,6 +92,8 @@ def lib():
         "nh_mean_rate_hz": (i32, [vp, P(f64)]),
         "nh_read_pn_total": (i32, [vp, vp, u32]),
         "nh_events_total": (i32, [vp, P(u64)]),
+        "nh_mark_counters": (i32, [vp]),
+        "nh_counts_since_mark": (i32, [vp, P(u64), P(u64)]),
         "nh_spike_row": (i32, [vp, u64, vp, u32]),
         "nh_last_spikes": (i32, [vp, vp, u32, P(u32)]),
         "nh_spike_counts": (i32, [vp, u64, vp, u32]),

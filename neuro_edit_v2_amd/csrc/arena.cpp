@@ -538,6 +538,7 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         TRY(alloc(d_bits, (size_t)bits_rows * std::max<uint32_t>(words, 1) * 8));
     }
     TRY(alloc(d_pn, (size_t)n_total * 4));
+    TRY(alloc(d_pn_mark, (size_t)n_total * 4));
     TRY(alloc(d_last, (size_t)n_total * 4));
     HIPCHK(hipMemset(d_last.p, 0xFF, d_last.bytes));
     TRY(alloc(d_net_spikes, (size_t)nn * 8));
@@ -1038,6 +1039,7 @@ int Arena::reset() {
     // (a failed paired launch is what a reset recovers from: no abort check)
     HIPCHK(hipStreamSynchronize(stream));
     timing_valid = false;
+    pn_marked = false;
     HIPCHK(hipMemcpyAsync(d_x.p, d_x0.p, d_x0.bytes, hipMemcpyDeviceToDevice, stream));
     HIPCHK(hipMemsetAsync(d_s.p, 0, d_s.bytes, stream));
     HIPCHK(hipMemsetAsync(d_ring.p, 0, d_ring.bytes, stream));
@@ -1182,6 +1184,38 @@ int Arena::events_total(uint64_t &out) {
     TRY(read_u32(d_pn, pn, n_total));
     for (uint32_t i = 0; i < n_total; ++i)
         if (pn[i]) out += (uint64_t)pn[i] * h_deg_[i];
+    return NH_OK;
+}
+
+int Arena::mark_counters() {
+    if (n_total == 0) {
+        pn_marked = true;
+        return NH_OK;
+    }
+    if (!d_pn_mark.p) {
+        // (allocated with the arena; an arena built elsewhere gets it here:
+        // the allocation's zero fill runs on the null stream, which the
+        // arena's non-blocking stream does not wait for, so wait for it)
+        TRY(alloc(d_pn_mark, (size_t)n_total * 4));
+        HIPCHK(hipDeviceSynchronize());
+    }
+    HIPCHK(hipMemcpyAsync(d_pn_mark.p, d_pn.p, (size_t)n_total * 4, hipMemcpyDeviceToDevice, stream));
+    pn_marked = true;
+    return NH_OK;
+}
+
+int Arena::counts_since_mark(uint64_t &spikes, uint64_t &events) {
+    spikes = events = 0;
+    if (!pn_marked) return set_error(NH_ERR_ARG, "no counter mark since creation or reset");
+    if (n_total == 0) return NH_OK;
+    std::vector<uint32_t> pn, mk;
+    TRY(read_u32(d_pn, pn, n_total));
+    TRY(read_u32(d_pn_mark, mk, n_total));
+    for (uint32_t i = 0; i < n_total; ++i) {
+        const uint32_t d = pn[i] - mk[i];
+        spikes += d;
+        events += (uint64_t)d * h_deg_[i];
+    }
     return NH_OK;
 }
 

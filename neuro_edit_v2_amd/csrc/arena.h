@@ -106,6 +106,10 @@ class Arena {
     int spikes_total(uint64_t &out);
     int spikes_in(uint32_t base, uint32_t n, uint64_t &out);
     int events_total(uint64_t &out);
+    // Counter snapshot taken on the stream (a device copy of pn_total, no
+    // host wait), and the spikes and events since it (synchronises).
+    int mark_counters();
+    int counts_since_mark(uint64_t &spikes, uint64_t &events);
     int drain_raster();   // device raster window -> host vectors
 
     // state
@@ -134,6 +138,8 @@ class Arena {
     DBuf d_t, d_nets, d_wave_net, d_x, d_x0, d_s, d_ring, d_ext, d_prow, d_ptab, d_p, d_mreg;
     DBuf d_stim_ptr, d_stim_input, d_stim_w, d_stim_win;
     DBuf d_spk, d_cnt, d_bits, d_pn, d_last, d_net_spikes;
+    DBuf d_pn_mark;            // pn_total at the last mark_counters
+    bool pn_marked = false;
     DBuf d_rast_bits, d_rast_ctl;
     DBuf d_src_ptr, d_syn, d_delay16, d_w, d_totals;
     DBuf d_watch, d_xrec;

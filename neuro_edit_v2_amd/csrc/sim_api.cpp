@@ -352,6 +352,21 @@ int nh_events_total(nh_sim *s, uint64_t *out) {
     return s->ar.events_total(*out);
 }
 
+int nh_mark_counters(nh_sim *s) {
+    return nh::guarded([&]() -> int {
+        CHECK_SIM(s);
+        return s->ar.mark_counters();
+    });
+}
+
+int nh_counts_since_mark(nh_sim *s, uint64_t *spikes, uint64_t *events) {
+    return nh::guarded([&]() -> int {
+        CHECK_SIM(s);
+        if (!spikes || !events) return set_error(NH_ERR_ARG, "null argument");
+        return s->ar.counts_since_mark(*spikes, *events);
+    });
+}
+
 int nh_mean_rate_hz(nh_sim *s, double *out) {
     CHECK_SIM(s);
     *out = 0.0;

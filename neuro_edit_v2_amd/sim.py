@@ -128,6 +128,17 @@ class Sim:
         L.check(L.lib().nh_events_total(self._h, C.byref(v)))
         return v.value
 
+    def mark_counters(self):
+        """Snapshot the per-neuron spike totals on the device, in stream
+        order and without waiting (measurement helper)."""
+        L.check(L.lib().nh_mark_counters(self._h))
+
+    def counts_since_mark(self):
+        """(spikes, synaptic events) since the last mark_counters."""
+        sp, ev = C.c_uint64(), C.c_uint64()
+        L.check(L.lib().nh_counts_since_mark(self._h, C.byref(sp), C.byref(ev)))
+        return sp.value, ev.value
+
     def mean_rate_hz(self):
         v = C.c_double()
         L.check(L.lib().nh_mean_rate_hz(self._h, C.byref(v)))

@@ -404,3 +404,26 @@ def test_custom_tau_net_matches_oracle():
     assert np.array_equal(c, rc) and np.array_equal(ids, ri)
     assert f32eq(g.v(), o.v())
     assert g.events_total() == o.events_total()
+
+
+@pytest.mark.parametrize("shape", [1, 2], ids=["fused", "paired"])
+def test_counter_mark_counts_the_window_exactly(shape):
+    """nh_mark_counters / nh_counts_since_mark (the benchmark's window
+    counters, a device-side snapshot): spikes and events since the mark equal
+    the difference of the totals read on the host; a reset clears the mark."""
+    from neuro_edit_v2_amd import _lib as L
+    net = N.cube_net(6000, mean_deg=80.0, seed=4, w_scale=0.7)
+    sim = Sim(net, SimConfig(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=30.0, noise_kick_mv=20.0,
+                             seed=3, int_weights=True, step_shape=shape))
+    sim.run_steps(51)
+    s0, e0 = sim.spikes_total(), sim.events_total()
+    sim.mark_counters()
+    sim.run_steps(77)
+    sp, ev = sim.counts_since_mark()
+    assert sp > 0 and sp == sim.spikes_total() - s0
+    assert ev == sim.events_total() - e0
+    sim.reset_state()
+    with pytest.raises(L.NHError):
+        sim.counts_since_mark()
+    sim.mark_counters()
+    assert sim.counts_since_mark() == (0, 0)
