@@ -82,14 +82,14 @@ def bench_batched(args, ring_int=True):
         tuned = {"chosen": rep.strip().splitlines()[-1].replace("chosen: ", ""),
                  "tune_s": round(time.perf_counter() - t_tune, 2)}
     b.run_steps(warm)                    # untimed warm-up (graph capture, first touches)
+    b.mark_counters()                    # window-start counters, taken on the device
     b.synchronize()
-    bev0 = b.events_total()
     t0 = time.perf_counter()
     b.run_steps(steps)
     b.synchronize()
     bwall = time.perf_counter() - t0
     bgpu = b.last_run_ms()
-    bev = b.events_total() - bev0
+    bev = b.counts_since_mark()[1]
     plain_wall, plain_ev, same, spikes = 0.0, 0, True, 0
     dev = []
     for k, net in enumerate(nets if not args.batch_no_plain else []):
@@ -97,13 +97,13 @@ def bench_batched(args, ring_int=True):
         for q in range(8):
             s.play_input(q, series[k][:, q])
         s.run_steps(warm)
+        s.mark_counters()
         s.synchronize()
-        pev0 = s.events_total()
         t1 = time.perf_counter()
         s.run_steps(steps)
         s.synchronize()
         plain_wall += time.perf_counter() - t1
-        plain_ev += s.events_total() - pev0
+        plain_ev += s.counts_since_mark()[1]
         _, pc, pi = s.read_spikes()
         _, bc, bi = b.read_spikes(k)
         same = same and np.array_equal(pc, bc) and np.array_equal(pi, bi)

@@ -299,6 +299,9 @@ int nh_batch_run_steps(nh_batch *b, uint64_t n_steps);
 int nh_batch_sync(nh_batch *b);
 int nh_batch_spikes_total(nh_batch *b, uint32_t net, uint64_t *out);
 int nh_batch_events_total(nh_batch *b, uint64_t *out);
+/* nh_mark_counters / nh_counts_since_mark over the whole batch (additions). */
+int nh_batch_mark_counters(nh_batch *b);
+int nh_batch_counts_since_mark(nh_batch *b, uint64_t *spikes, uint64_t *events);
 int nh_batch_read_state(nh_batch *b, uint32_t net, uint32_t slot, float *out, uint32_t n);
 int nh_batch_read_spikes(nh_batch *b, uint32_t net, uint64_t *step0, uint32_t *counts,
                          uint64_t max_steps, uint32_t *ids, uint64_t max_ids,

@@ -139,6 +139,8 @@ def lib():
         "nh_batch_sync": (i32, [vp]),
         "nh_batch_spikes_total": (i32, [vp, u32, P(u64)]),
         "nh_batch_events_total": (i32, [vp, P(u64)]),
+        "nh_batch_mark_counters": (i32, [vp]),
+        "nh_batch_counts_since_mark": (i32, [vp, P(u64), P(u64)]),
         "nh_batch_read_state": (i32, [vp, u32, u32, vp, u32]),
         "nh_batch_read_spikes": (i32, [vp, u32, P(u64), vp, u64, vp, u64, P(u64), P(u64)]),
         "nh_batch_play_input": (i32, [vp, u32, u32, vp, u64]),

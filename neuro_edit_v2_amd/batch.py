@@ -62,6 +62,16 @@ class Batch:
         L.check(L.lib().nh_batch_events_total(self._h, C.byref(v)))
         return v.value
 
+    def mark_counters(self):
+        """Snapshot the spike totals on the device, without waiting."""
+        L.check(L.lib().nh_batch_mark_counters(self._h))
+
+    def counts_since_mark(self):
+        """(spikes, synaptic events) of the whole batch since the last mark."""
+        sp, ev = C.c_uint64(), C.c_uint64()
+        L.check(L.lib().nh_batch_counts_since_mark(self._h, C.byref(sp), C.byref(ev)))
+        return sp.value, ev.value
+
     def state(self, net, slot):
         out = np.zeros(max(self.n[net], 1), np.float32)
         L.check(L.lib().nh_batch_read_state(self._h, net, slot, L.ptr(out), self.n[net]))

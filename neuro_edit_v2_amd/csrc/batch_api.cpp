@@ -164,6 +164,21 @@ int nh_batch_events_total(nh_batch *b, uint64_t *out) {
     return b->ar.events_total(*out);
 }
 
+int nh_batch_mark_counters(nh_batch *b) {
+    return nh::guarded([&]() -> int {
+        CHECK_B(b);
+        return b->ar.mark_counters();
+    });
+}
+
+int nh_batch_counts_since_mark(nh_batch *b, uint64_t *spikes, uint64_t *events) {
+    return nh::guarded([&]() -> int {
+        CHECK_B(b);
+        if (!spikes || !events) return set_error(NH_ERR_ARG, "null argument");
+        return b->ar.counts_since_mark(*spikes, *events);
+    });
+}
+
 int nh_batch_read_state(nh_batch *b, uint32_t net, uint32_t slot, float *out, uint32_t n) {
     CHECK_B(b);
     if (net >= b->ar.n_nets || slot >= 2) return set_error(NH_ERR_ARG, "net/slot out of range");
