@@ -810,7 +810,8 @@ int Arena::check_pair_abort() {
     uint32_t f = 0;
     HIPCHK(hipMemcpy(&f, d_pair_abort.p, 4, hipMemcpyDeviceToHost));
     if (!f) return NH_OK;
-    HIPCHK(hipMemset(d_pair_abort.p, 0, 4));
+    HIPCHK(hipMemsetAsync(d_pair_abort.p, 0, 4, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     return set_error(NH_ERR_HIP, "paired-step launch: a wave's wait for its delay-1 sources ran out "
                                  "(workgroups not co-resident); the state is invalid, reset the simulation");
 }
@@ -1192,13 +1193,7 @@ int Arena::mark_counters() {
         pn_marked = true;
         return NH_OK;
     }
-    if (!d_pn_mark.p) {
-        // (allocated with the arena; an arena built elsewhere gets it here:
-        // the allocation's zero fill runs on the null stream, which the
-        // arena's non-blocking stream does not wait for, so wait for it)
-        TRY(alloc(d_pn_mark, (size_t)n_total * 4));
-        HIPCHK(hipDeviceSynchronize());
-    }
+    if (!d_pn_mark.p) TRY(alloc(d_pn_mark, (size_t)n_total * 4));  // (an arena built elsewhere)
     HIPCHK(hipMemcpyAsync(d_pn_mark.p, d_pn.p, (size_t)n_total * 4, hipMemcpyDeviceToDevice, stream));
     pn_marked = true;
     return NH_OK;

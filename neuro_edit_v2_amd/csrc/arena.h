@@ -51,7 +51,13 @@ struct DBuf {
             bytes = 0;
             return e;
         }
-        return hipMemset(p, 0, bytes);
+        e = hipMemset(p, 0, bytes);
+        if (e != hipSuccess) return e;
+        // the zero fill runs on the null stream, which the arenas'
+        // non-blocking streams do not wait for: it is finished before the
+        // buffer is handed out (a buffer allocated after creation is first
+        // touched by work on those streams)
+        return hipStreamSynchronize(nullptr);
     }
     template <class T> T *as() const { return (T *)p; }
 };
