@@ -150,7 +150,10 @@ int nh_spike_row(nh_sim *s, uint64_t back, uint64_t *words, uint32_t n_words);
 /* Backend::last_spikes -- lib.rs:142 (ascending ids) */
 int nh_last_spikes(nh_sim *s, uint32_t *ids, uint32_t cap, uint32_t *n_out);
 /* Backend::spike_counts(rows) -- lib.rs:146; NH_ERR_WINDOW for the
- * "exceeds the recorded steps" error (cpu.rs:463-468). */
+ * "exceeds the recorded steps" error (cpu.rs:463-468).  Replaces
+ * CudaSim::spike_counts (cuda.rs:725-763, which copies the whole record
+ * ring): the window is counted on the device (a count buffer of n u32 is
+ * allocated at the first windowed call) and only n counts are copied. */
 int nh_spike_counts(nh_sim *s, uint64_t rows, uint32_t *out, uint32_t n);
 
 /* Backend::set_input_constant / input_values / play_input / loop_input --
@@ -308,6 +311,10 @@ int nh_batch_read_spikes(nh_batch *b, uint32_t net, uint64_t *step0, uint32_t *c
                          uint64_t *n_steps, uint64_t *n_ids);
 int nh_batch_play_input(nh_batch *b, uint32_t net, uint32_t input, const float *series,
                         uint64_t len);
+/* nh_spike_counts for one packed network (Backend::spike_counts, lib.rs:146;
+ * cuda.rs:725-763): record_spikes must be set in the first network's config
+ * (the record ring is arena-wide); local neuron ids. */
+int nh_batch_spike_counts(nh_batch *b, uint32_t net, uint64_t rows, uint32_t *out, uint32_t n);
 int nh_batch_last_run_ms(nh_batch *b, double *ms);
 /* nh_autotune for a batch (from step 0; resets the batch). */
 int nh_batch_autotune(nh_batch *b, uint32_t trial_steps, char *report, uint32_t report_cap);

@@ -142,6 +142,7 @@ def lib():
         "nh_batch_mark_counters": (i32, [vp]),
         "nh_batch_counts_since_mark": (i32, [vp, P(u64), P(u64)]),
         "nh_batch_read_state": (i32, [vp, u32, u32, vp, u32]),
+        "nh_batch_spike_counts": (i32, [vp, u32, u64, vp, u32]),
         "nh_batch_read_spikes": (i32, [vp, u32, P(u64), vp, u64, vp, u64, P(u64), P(u64)]),
         "nh_batch_play_input": (i32, [vp, u32, u32, vp, u64]),
         "nh_batch_last_run_ms": (i32, [vp, P(f64)]),

@@ -77,6 +77,18 @@ class Batch:
         L.check(L.lib().nh_batch_read_state(self._h, net, slot, L.ptr(out), self.n[net]))
         return out[: self.n[net]]
 
+    def spike_counts(self, net, rows):
+        """Per-neuron spike counts of network `net` over its last `rows`
+        steps (Backend::spike_counts); ValueError for a window between the
+        record ring and t."""
+        out = np.zeros(max(self.n[net], 1), np.uint32)
+        rows = min(int(rows), (1 << 64) - 1)
+        rc = L.lib().nh_batch_spike_counts(self._h, net, rows, L.ptr(out), self.n[net])
+        if rc == L.NH_ERR_WINDOW:
+            raise ValueError(L.lib().nh_last_error().decode())
+        L.check(rc)
+        return out[: self.n[net]]
+
     def read_spikes(self, net):
         lib = L.lib()
         s0, ns, ni = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -533,7 +533,7 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     TRY(alloc(d_spk, (size_t)3 * std::max<uint32_t>(n_total, 1) * 4));
     TRY(alloc(d_cnt, 4 * 4));
     bits_rows = 0;
-    if (nn == 1 && cfgs[0].record_spikes) {
+    if (cfgs[0].record_spikes) {
         bits_rows = std::max<uint32_t>(cfgs[0].spike_record_steps, 2);
         TRY(alloc(d_bits, (size_t)bits_rows * std::max<uint32_t>(words, 1) * 8));
     }
@@ -1155,6 +1155,24 @@ int Arena::read_u64(const DBuf &b, std::vector<uint64_t> &out, size_t n) {
     out.resize(n);
     if (n) HIPCHK(hipMemcpy(out.data(), b.p, n * 8, hipMemcpyDeviceToHost));
     return NH_OK;
+}
+
+int Arena::count_rows(uint32_t base, uint32_t n, uint64_t rows, uint32_t *dev_out, uint32_t *host_out) {
+    if (n == 0) return NH_OK;
+    if (!d_bits.p || rows == 0 || rows > bits_rows || rows >= t || base % 64u ||
+        (uint64_t)base + ((n + 63u) & ~63u) > n_total)
+        return set_error(NH_ERR_ARG, "count_rows: window or range outside the record ring");
+    launch_count_rows(d_bits.as<uint64_t>(), bits_rows, words, base, n, t, (uint32_t)rows, dev_out, stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host_out, dev_out, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+    return sync();
+}
+
+int Arena::read_pn(uint32_t base, uint32_t n, uint32_t *host_out) {
+    if (n == 0) return NH_OK;
+    if ((uint64_t)base + n > n_total) return set_error(NH_ERR_ARG, "read_pn: range outside the arena");
+    HIPCHK(hipMemcpyAsync(host_out, d_pn.as<uint32_t>() + base, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+    return sync();
 }
 
 // Totals follow from the per-neuron lifetime counts: spikes = sum pn,

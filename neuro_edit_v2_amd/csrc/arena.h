@@ -26,6 +26,8 @@ void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mm
                  bool lean, bool w8, hipStream_t st);
 uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool lean, bool w8,
                              uint32_t block);
+void launch_count_rows(const uint64_t *bits, uint32_t R, uint32_t words, uint32_t base, uint32_t n,
+                       uint64_t t, uint32_t rows, uint32_t *out, hipStream_t st);
 
 int set_error(int code, const std::string &msg);
 
@@ -117,6 +119,14 @@ class Arena {
     int mark_counters();
     int counts_since_mark(uint64_t &spikes, uint64_t &events);
     int drain_raster();   // device raster window -> host vectors
+    // Per-neuron spike counts of arena neurons [base, base + n) over the
+    // `rows` steps before t (0 < rows <= bits_rows, rows < t), counted on the
+    // device into dev_out (n u32 of device memory) and copied to host_out;
+    // synchronises.
+    int count_rows(uint32_t base, uint32_t n, uint64_t rows, uint32_t *dev_out, uint32_t *host_out);
+    // Lifetime spike counts of arena neurons [base, base + n) into host_out.
+    int read_pn(uint32_t base, uint32_t n, uint32_t *host_out);
+    DBuf d_counts;             // count_rows output (allocated at first use)
 
     // state
     uint32_t n_nets = 0, n_total = 0, D = 2, words = 0, bits_rows = 0;

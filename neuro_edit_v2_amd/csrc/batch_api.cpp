@@ -213,6 +213,33 @@ int nh_batch_read_spikes(nh_batch *b, uint32_t net, uint64_t *step0, uint32_t *c
     });
 }
 
+// Backend::spike_counts (sim/src/lib.rs:119-184; cuda.rs:725-763) of one
+// packed network: lifetime counts when rows >= t, the window counted on the
+// device when rows <= the record ring, NH_ERR_WINDOW between the two.
+int nh_batch_spike_counts(nh_batch *b, uint32_t net, uint64_t rows, uint32_t *out, uint32_t n) {
+    return nh::guarded([&]() -> int {
+        CHECK_B(b);
+        if (net >= b->ar.n_nets) return set_error(NH_ERR_ARG, "net out of range");
+        if (!out) return set_error(NH_ERR_ARG, "null output");
+        const auto &sl = b->ar.slots[net];
+        if (n < sl.n) return set_error(NH_ERR_ARG, "output too small");
+        std::memset(out, 0, (size_t)sl.n * 4);
+        const uint64_t t = b->ar.t;
+        if (!b->cfgs[0].record_spikes || t == 0 || sl.n == 0 || rows == 0) return NH_OK;
+        if (rows >= t) return b->ar.read_pn(sl.base, sl.n, out);
+        const uint32_t R = b->ar.bits_rows;
+        if (rows > R)
+            return set_error(NH_ERR_WINDOW, "spike_counts window " + std::to_string(rows) +
+                                                " exceeds the recorded " + std::to_string(R) + " steps");
+        if (!b->ar.d_counts.p) {
+            const hipError_t e = b->ar.d_counts.alloc((size_t)b->ar.n_total * 4);
+            if (e != hipSuccess) return set_error(NH_ERR_HIP, std::string("count buffer: ") + hipGetErrorString(e));
+            b->ar.dev_bytes += b->ar.d_counts.bytes;
+        }
+        return b->ar.count_rows(sl.base, sl.n, rows, b->ar.d_counts.as<uint32_t>(), out);
+    });
+}
+
 int nh_batch_play_input(nh_batch *b, uint32_t net, uint32_t input, const float *series, uint64_t len) {
     CHECK_B(b);
     if (net >= b->ar.n_nets || input >= b->ar.slots[net].n_inputs)

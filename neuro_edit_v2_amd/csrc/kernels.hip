@@ -749,6 +749,47 @@ __global__ void k_watch(StepArgs a) {
     a.xrec[(size_t)(t % a.xrec_depth) * a.n_watch + w] = a.x[a.watch[w]];
 }
 
+// Per-neuron spike counts over the record ring (sim/src/cuda.rs:725-763
+// counts on the host after copying the whole ring): arena neurons
+// [base, base + n), base a multiple of 64, over the `rows` steps before step
+// t; ring of R rows, one 64-bit word per wave and step.  One thread per
+// neuron; a wave's word of a row is one wave-uniform load, eight rows in
+// flight.  Only n counts leave the device.
+__global__ void k_count_rows(const uint64_t *bits, uint32_t R, uint32_t words, uint32_t base,
+                             uint32_t n, uint64_t t, uint32_t rows, uint32_t *out) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    // whole waves past the range leave (their word index would pass the row)
+    if (uniform(j & ~63u) >= n) return;
+    const uint32_t i = base + j;
+    const uint32_t wi = uniform(i >> 6);
+    const uint32_t bit = i & 63u;
+    uint32_t c = 0;
+    uint32_t row = (uint32_t)((t - 1u) % R);
+    uint32_t b = 0;
+    for (; b + 8u <= rows; b += 8u) {
+        uint64_t w[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8u; ++q) {
+            w[q] = bits[(size_t)row * words + wi];
+            row = row == 0u ? R - 1u : row - 1u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 8u; ++q) c += (uint32_t)(w[q] >> bit) & 1u;
+    }
+    for (; b < rows; ++b) {
+        c += (uint32_t)(bits[(size_t)row * words + wi] >> bit) & 1u;
+        row = row == 0u ? R - 1u : row - 1u;
+    }
+    if (j < n) out[j] = c;
+}
+
+void launch_count_rows(const uint64_t *bits, uint32_t R, uint32_t words, uint32_t base, uint32_t n,
+                       uint64_t t, uint32_t rows, uint32_t *out, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_count_rows, dim3((n + 255) / 256), dim3(256), 0, st, bits, R, words, base, n, t,
+                       rows, out);
+}
+
 // Served arena: a newly admitted range starts with empty ring columns,
 // zero current and counters, and no last spike.
 __global__ void k_clear_range(StepArgs a, uint32_t base, uint32_t len) {

@@ -430,38 +430,29 @@ int nh_last_spikes(nh_sim *s, uint32_t *ids, uint32_t cap, uint32_t *n_out) {
     return NH_OK;
 }
 
-// CpuSim::spike_counts -- cpu.rs:452-486
+// CpuSim::spike_counts -- cpu.rs:452-486; CudaSim::spike_counts
+// (cuda.rs:725-763) copies the whole record ring and counts on the host,
+// here the window is counted on the device and only n counts are copied.
 int nh_spike_counts(nh_sim *s, uint64_t rows, uint32_t *out, uint32_t n) {
-    CHECK_SIM(s);
-    if (n < s->n) return set_error(NH_ERR_ARG, "output too small");
-    std::memset(out, 0, (size_t)s->n * 4);
-    const uint64_t t = s->ar.t;
-    if (!s->cfg.record_spikes || t == 0) return NH_OK;
-    if (rows >= t) {
-        std::vector<uint32_t> v;
-        TRY(s->ar.read_u32(s->ar.d_pn, v, s->n));
-        std::memcpy(out, v.data(), (size_t)s->n * 4);
-        return NH_OK;
-    }
-    const uint32_t R = s->ar.bits_rows;
-    if (rows > R)
-        return set_error(NH_ERR_WINDOW, "spike_counts window " + std::to_string(rows) +
-                                            " exceeds the recorded " + std::to_string(R) + " steps");
-    std::vector<uint64_t> bits;
-    TRY(s->ar.read_u64(s->ar.d_bits, bits, (size_t)R * s->ar.words));
-    for (uint64_t back = 0; back < rows && back < t; ++back) {
-        const uint64_t row = (t - 1 - back) % R;
-        for (uint32_t wi = 0; wi < s->ar.words; ++wi) {
-            uint64_t w = bits[row * s->ar.words + wi];
-            while (w) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(w);
-                w &= w - 1;
-                const uint32_t i = wi * 64 + b;
-                if (i < s->n) out[i] += 1;
-            }
+    return nh::guarded([&]() -> int {
+        CHECK_SIM(s);
+        if (!out) return set_error(NH_ERR_ARG, "null output");
+        if (n < s->n) return set_error(NH_ERR_ARG, "output too small");
+        std::memset(out, 0, (size_t)s->n * 4);
+        const uint64_t t = s->ar.t;
+        if (!s->cfg.record_spikes || t == 0 || s->n == 0 || rows == 0) return NH_OK;
+        if (rows >= t) return s->ar.read_pn(0, s->n, out);
+        const uint32_t R = s->ar.bits_rows;
+        if (rows > R)
+            return set_error(NH_ERR_WINDOW, "spike_counts window " + std::to_string(rows) +
+                                                " exceeds the recorded " + std::to_string(R) + " steps");
+        if (!s->ar.d_counts.p) {
+            const hipError_t e = s->ar.d_counts.alloc((size_t)s->ar.n_total * 4);
+            if (e != hipSuccess) return set_error(NH_ERR_HIP, std::string("count buffer: ") + hipGetErrorString(e));
+            s->ar.dev_bytes += s->ar.d_counts.bytes;
         }
-    }
-    return NH_OK;
+        return s->ar.count_rows(0, s->n, rows, s->ar.d_counts.as<uint32_t>(), out);
+    });
 }
 
 int nh_set_input_constant(nh_sim *s, uint32_t input, float value) {

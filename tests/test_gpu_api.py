@@ -427,3 +427,59 @@ def test_counter_mark_counts_the_window_exactly(shape):
         sim.counts_since_mark()
     sim.mark_counters()
     assert sim.counts_since_mark() == (0, 0)
+
+
+@pytest.mark.parametrize("shape", [1, 2], ids=["fused", "paired"])
+def test_spike_counts_counted_on_device_match_oracle(shape):
+    """Backend::spike_counts (cuda.rs:725-763; cpu.rs:452-486) with the
+    window counted on the device: rows within the record ring (every
+    remainder of the kernel's eight-row unroll, windows that wrap the
+    ring), rows >= t (lifetime counts) and the error between the two."""
+    net = N.cube_net(5000, mean_deg=60.0, seed=12)
+    kw = dict(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=40.0, seed=3, int_weights=True,
+              record_spikes=True, spike_record_steps=100)
+    g, o = pair_sims(net, shape=shape, **kw)
+    for steps in (5, 233, 120):
+        g.run_steps(steps)
+        o.run_steps(steps)
+        t = o.t
+        for rows in (1, 2, 7, 8, 9, 15, 16, 17, 63, 64, 99, 100, t, t + 1, 1 << 40):
+            if rows > 100 and rows < t:
+                continue
+            assert np.array_equal(g.spike_counts(rows), o.spike_counts(rows)), (t, rows)
+        for rows in (101, t - 1):
+            if 100 < rows < t:
+                with pytest.raises(ValueError, match="exceeds"):
+                    g.spike_counts(rows)
+    assert int(o.spike_counts(100).sum()) > 100
+
+
+def test_batch_spike_counts_per_network_match_oracle():
+    """nh_batch_spike_counts: each packed network's window counts equal the
+    oracle's for that network alone (int ring)."""
+    from neuro_edit_v2_amd import workloads as W
+    from neuro_edit_v2_amd.batch import Batch
+    sizes = [257, 1000, 64, 3000, 700]
+    steps = 160
+    nets, series, cfgs = [], [], []
+    for k, n in enumerate(sizes):
+        nets.append(N.grid_net(0, model=k % 2, seed=k + 3, dt_ms=0.5, dims=W.grid_dims(n)))
+        series.append(W.stream_window(W.rng_stream(100 + k, 8, 0.05), 8, steps))
+        cfgs.append(SimConfig(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=30.0, seed=k + 1,
+                              int_weights=True, record_spikes=True, spike_record_steps=64))
+    b = Batch(nets, cfgs)
+    for k in range(len(nets)):
+        for q in range(8):
+            b.play_input(k, q, series[k][:, q])
+    b.run_steps(steps)
+    for k, net in enumerate(nets):
+        o = O.OracleSim(net, O.default_cfg(dt_ms=0.5, synapse_tau_ms=5.0, noise_rate_hz=30.0,
+                                           seed=k + 1, int_weights=1, exp_mode=1,
+                                           record_spikes=1, spike_record_steps=64))
+        for q in range(8):
+            o.play_input(q, series[k][:, q])
+        o.run_steps(steps)
+        for rows in (1, 13, 64, steps, 1 << 40):
+            assert np.array_equal(b.spike_counts(k, rows), o.spike_counts(rows)), (k, rows)
+        with pytest.raises(ValueError, match="exceeds"):
+            b.spike_counts(k, 65)

@@ -57,6 +57,33 @@ def test_reference_fingerprint_bit_exact():
     assert h == 0x9D792BD233132F5D
 
 
+BENCH_PUBLISHED = {1: 2448025, 0: 2448300}   # NOTES-for-data.md:2398-2399 (int, float ring)
+
+
+@pytest.mark.parametrize("ring", [1, 0])
+def test_reference_bench_spike_total(ring):
+    """The reference's own benchmark regime (NOTES-for-data.md:24-30,
+    2391-2399): bench_net(10000, K=100, seed 1, wscale 0.7, w-inh 5), dt 0.1,
+    synaptic tau 2 ms (SimConfig defaults, state.rs:35-49), Poisson kicks
+    500 Hz x 20 mV, seed 1, int ring at scale 4096 or the float ring;
+    run_bench's lifetime total over a 100 ms warm-up plus 3000 ms
+    (bench.rs:63-89).  The reference published 2,448,025 (int) and
+    2,448,300 (float) spikes on its CUDA backend, whose NVRTC build
+    contracts multiply-adds, so the totals are a +-0.5 % band, not exact
+    bits.  A wrong delay rounding, decay, kick rule or ring index moves the
+    total of this 21 Hz regime far outside it."""
+    b = O.NetBuilder()
+    b.bench_net(10000, 100, 1, 0.7, 5.0)
+    net = b.build(0.1)
+    sim = O.OracleSim(net, O.default_cfg(dt_ms=0.1, synapse_tau_ms=2.0, noise_rate_hz=500.0,
+                                         noise_kick_mv=20.0, seed=1, int_weights=ring,
+                                         int_scale=4096.0))
+    sim.run_ms(100.0)
+    sim.run_ms(3000.0)
+    total = sim.spikes_total()
+    assert abs(total - BENCH_PUBLISHED[ring]) <= 0.005 * BENCH_PUBLISHED[ring], total
+
+
 # ---------------------------------------------------------------- noise
 def splitmix64(x):
     M = (1 << 64) - 1
@@ -482,7 +509,9 @@ TONIC = [281.0, 30.0, -70.6, -50.4, 2.0, 4.0, 144.0, 80.5, -70.6, 20.0]
 
 def test_adex_init_and_fires_with_reset():
     """sim/tests/adex.rs:35-54 (init [EL, 0]) and :56-121 (every driven
-    AdEx neuron fires repeatedly -- impossible without a working reset)."""
+    AdEx neuron fires repeatedly).  Repeated firing alone does not show the
+    reset (without one, v sits at the Vpeak cut and fires every step); the
+    closed-form steps below pin it."""
     b = netb_line(4)
     for i in (1, 3):
         b.set_model(i, 1, TONIC)
@@ -496,6 +525,98 @@ def test_adex_init_and_fires_with_reset():
         c = sim.spike_counts(1 << 62)
         assert c[1] >= 2 and c[3] >= 2 and c[0] >= 2
         assert np.all(sim.v()[[1, 3]] <= TONIC[9])  # Vpeak cut
+
+
+def adex_one(x0, dt=0.1, ext=0.0, **cfg):
+    """One TONIC AdEx neuron started from state x0 = (v, w), one step, with
+    external current ext (pA); returns (v, w) after the step."""
+    b = netb_line(1)
+    b.set_model(0, 1, TONIC)
+    net = b.build(dt)
+    net["x0"] = np.array(x0, np.float32)
+    sim = O.OracleSim(net, O.default_cfg(dt_ms=dt, **cfg))
+    sim.set_external(0, ext)
+    sim.run_steps(1)
+    return float(sim.state(0)[0]), float(sim.state(1)[0])
+
+
+def adex_closed(v, w, I, dt, clamp=10.0):
+    """adex.rs:163-176 evaluated in f64: reset at entry, v with the clamped
+    exponential, w with the NEW v, Vpeak cut."""
+    C, gL, EL, VT, dT, a, tw, b, Vr, Vp = TONIC
+    if v >= Vp:
+        v, w = Vr, w + b
+    ex = math.exp(min((v - VT) / dT, clamp))
+    v = v + dt * (-gL * (v - EL) + gL * dT * ex - w + I) / C
+    w = w + dt * (a * (v - EL) - w) / tw
+    return min(v, Vp), w, v >= Vp
+
+
+def test_adex_reset_at_step_entry():
+    """adex.rs:164-167: a neuron entering the step at v >= Vpeak starts the
+    step from v = Vr and w + b.  Without the reset v would integrate from
+    Vpeak (the exponential clamps at 10, v jumps past Vpeak and stays cut
+    there) and w would not gain b."""
+    for w0 in (0.0, 37.5):
+        v, w = adex_one((TONIC[9], w0))
+        ev, ew, sp = adex_closed(TONIC[9], w0, 0.0, 0.1)
+        assert not sp
+        assert abs(v - ev) < 1e-4 and abs(w - ew) < 1e-4 * abs(ew)
+        assert v < -70.0 and w > w0 + 0.9 * TONIC[7]
+
+
+def test_adex_w_uses_the_new_v():
+    """adex.rs:170-171: w integrates with the v of this step.  From v = EL,
+    w = 0 and I > 0, v rises above EL, so w = dt*a*(v_new - EL)/tau_w > 0;
+    with the old v the same step leaves w exactly 0."""
+    C, gL, EL = TONIC[:3]
+    for I in (200.0, 800.0):
+        v, w = adex_one((EL, 0.0), ext=I)
+        ev, ew, _ = adex_closed(EL, 0.0, I, 0.1)
+        assert v > EL and w > 0.0
+        assert abs(v - ev) < 1e-4 and abs(w - ew) < 1e-3 * ew
+
+
+def test_adex_exponent_clamps_at_10():
+    """adex.rs:168: exp(min((v - VT)/dT, 10)).  From v = -20 mV,
+    (v - VT)/dT = 15.2: with the clamp a 1 us step moves v by about
+    +4.7 mV; without it by about +850 mV (past Vpeak, a spike)."""
+    v, w = adex_one((-20.0, 0.0), dt=0.001)
+    ev, _, sp = adex_closed(-20.0, 0.0, 0.0, 0.001)
+    assert not sp and abs(v - ev) < 1e-3
+    assert 4.0 < v + 20.0 < 5.5
+    free, _, sp_free = adex_closed(-20.0, 0.0, 0.0, 0.001, clamp=1e9)
+    assert sp_free and free == TONIC[9]
+
+
+def test_adex_vpeak_cut_exact():
+    """adex.rs:172-175: the step that crosses Vpeak stores v = Vpeak
+    exactly (the Euler value overshoots far above it) and reports the spike;
+    the next step starts from Vr."""
+    b = netb_line(1)
+    b.set_model(0, 1, TONIC)
+    net = b.build(0.1)
+    net["x0"] = np.array([-30.0, 0.0], np.float32)   # exp term already large
+    sim = O.OracleSim(net, O.default_cfg(record_spikes=1))
+    sim.run_steps(1)
+    _, _, sp = adex_closed(-30.0, 0.0, 0.0, 0.1)
+    assert sp and sim.spike_row(0).any()
+    assert float(sim.v()[0]) == TONIC[9]
+    sim.run_steps(1)
+    assert float(sim.v()[0]) < -70.0
+
+
+def test_adex_kick_gain_is_C():
+    """adex.rs:187-195 with cpu.rs:240-245: a kick adds kick_mv/dt * C pA,
+    so from v = EL one kicked step moves v by kick_mv plus the leak-free
+    exponential term dt*gL*dT*exp((EL - VT)/dT)/C.  (Rate*dt/1000 >= 1
+    makes the kick fire every step.)  Without the gain v would move by
+    kick_mv/C, 0.07 mV."""
+    C, gL, EL, VT, dT = TONIC[:5]
+    for kick in (20.0, 0.8):
+        v, _ = adex_one((EL, 0.0), noise_rate_hz=20000.0, noise_kick_mv=kick)
+        want = EL + kick + 0.1 * gL * dT * math.exp((EL - VT) / dT) / C
+        assert abs(v - want) < 1e-4 * max(1.0, kick)
 
 
 def test_adex_portable_exp_close_to_libm():
