@@ -47,11 +47,10 @@ def test_headline_int_raster_bit_exact_full_run(cube100k, scale):
 def test_headline_float_ring_tolerances(cube100k):
     """Float ring at 100k: the atomic order of float ring sums varies, so
     the bar is the stated tolerance.  Over the first 200 steps, v, u and the
-    synaptic current I of every neuron whose spike count still equals the
-    oracle's agree within 1e-4 relative (denominator max(|x|, 1)), and at
-    most 0.1 % of neurons have a different count; over 1 s the per-neuron
-    spike counts agree within 2 % mean absolute deviation.  The f32 vs FP64
-    oracle deviation is printed beside it."""
+    synaptic current I of every neuron agree within 1e-4 relative
+    (denominator max(|x|, 1)); over 1 s the per-neuron spike counts agree
+    within 2 % mean absolute deviation.  The FP64 oracle's deviation from
+    the f32 one is printed beside it."""
     kw = dict(HEAD, int_weights=False)
     print("case: 100k float ring", flush=True)
     sim = Sim(cube100k, SimConfig(**kw))
@@ -69,13 +68,12 @@ def test_headline_float_ring_tolerances(cube100k):
                           (sim.syn_current(), ref.syn_current(), ref64.syn_current())):
             g = g.astype(np.float64)
             rel = np.abs(g - r) / np.maximum(np.abs(r), 1.0)
-            worst = max(worst, float(rel[same].max()))
+            worst = max(worst, float(rel.max()))
             rel64 = np.abs(r - r64) / np.maximum(np.abs(r64), 1.0)
-            worst64 = max(worst64, float(rel64[same].max()))
+            worst64 = max(worst64, float(rel64.max()))
     print(f"200 steps: max rel v/u/I error vs f32 oracle {worst:.2e} (f32 vs f64 oracle "
           f"{worst64:.2e}); neurons with a different spike count: {diverged}")
     assert worst < 1e-4
-    assert diverged <= 100
     sim.run_steps(1800)
     ref.run_steps(1800)
     ref64.run_steps(1800)

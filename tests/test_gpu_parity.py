@@ -80,10 +80,10 @@ def test_int_runs_bitwise_deterministic(cube10k):
 
 @pytest.mark.parametrize("shape", [0, 1], ids=["two_kernel", "fused"])
 def test_float_ring_traces_and_counts(cube10k, shape):
-    """Float ring: atomic order varies, so the bar is statistical --
-    v/u/s within 1e-4 relative over the first steps, per-neuron spike
-    counts over 1 s within 2 % mean absolute deviation; both also stated
-    against the FP64 oracle."""
+    """Float ring: atomic order varies, so the bar is a tolerance -- v/u of
+    every neuron within 1e-4 relative (denominator max(|x|, 1)) over the
+    first 200 steps, per-neuron spike counts over 1 s within 2 % mean
+    absolute deviation; the deviation from the FP64 oracle is printed."""
     kw = dict(CFG0)
     sim = Sim(cube10k, SimConfig(step_shape=shape, exec_mode=1, **kw))
     ref = O.OracleSim(cube10k, O.default_cfg(**kw))
@@ -97,8 +97,8 @@ def test_float_ring_traces_and_counts(cube10k, shape):
             g = sim.state(slot).astype(np.float64)
             r = ref.state(slot)
             rel = np.abs(g - r) / np.maximum(np.abs(r), 1.0)
-            worst = max(worst, float(np.quantile(rel, 0.999)))
-    print(f"float ring, 200 steps: 99.9th pct relative v/u error vs f32 oracle {worst:.2e}")
+            worst = max(worst, float(rel.max()))
+    print(f"float ring, 200 steps: max relative v/u error vs f32 oracle {worst:.2e}")
     assert worst < 1e-4
     sim.run_steps(1800)
     ref.run_steps(1800)
@@ -110,7 +110,7 @@ def test_float_ring_traces_and_counts(cube10k, shape):
     mad64 = np.abs(r - r64).mean() / max(r64.mean(), 1e-9)
     print(f"1 s spike counts: MAD gpu-vs-f32 oracle {mad:.3%}, f32-vs-f64 oracle {mad64:.3%}, "
           f"totals gpu {g.sum():.0f} f32 {r.sum():.0f} f64 {r64.sum():.0f}")
-    assert mad < 0.02 or abs(g.sum() - r.sum()) / r.sum() < 0.02
+    assert mad < 0.02
 
 
 def test_smoke_entry():
