@@ -155,6 +155,13 @@ int nh_last_spikes(nh_sim *s, uint32_t *ids, uint32_t cap, uint32_t *n_out);
  * ring): the window is counted on the device (a count buffer of n u32 is
  * allocated at the first windowed call) and only n counts are copied. */
 int nh_spike_counts(nh_sim *s, uint64_t rows, uint32_t *out, uint32_t n);
+/* nh_spike_counts that allocates no device memory -- no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The window is
+ * counted into a 16 MiB library scratch per GPU (at most 4,194,304 neurons;
+ * larger networks get NH_ERR_CAPACITY).  Calls that overlap on one GPU race
+ * on that scratch. */
+int nh_spike_counts_noalloc(nh_sim *s, uint64_t rows, uint32_t *out, uint32_t n);
 
 /* Backend::set_input_constant / input_values / play_input / loop_input --
  * lib.rs:152-163 (stimulus program kept host-side as in cuda.rs:241-252) */

@@ -97,6 +97,7 @@ def lib():
         "nh_spike_row": (i32, [vp, u64, vp, u32]),
         "nh_last_spikes": (i32, [vp, vp, u32, P(u32)]),
         "nh_spike_counts": (i32, [vp, u64, vp, u32]),
+        "nh_spike_counts_noalloc": (i32, [vp, u64, vp, u32]),
         "nh_set_input_constant": (i32, [vp, u32, f32]),
         "nh_input_values": (i32, [vp, vp, u32]),
         "nh_play_input": (i32, [vp, u32, vp, u64]),

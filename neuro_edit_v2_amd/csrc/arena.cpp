@@ -233,6 +233,7 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     stim_cols = cols;
     packed = (n_total <= (1u << 27) && D <= 32) ? 1 : 0;
 
+    HIPCHK(hipGetDevice(&device));
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));

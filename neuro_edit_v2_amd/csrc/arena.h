@@ -28,6 +28,10 @@ uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool l
                              uint32_t block);
 void launch_count_rows(const uint64_t *bits, uint32_t R, uint32_t words, uint32_t base, uint32_t n,
                        uint64_t t, uint32_t rows, uint32_t *out, hipStream_t st);
+// Words of the file-scope count scratch (nh_spike_counts_noalloc), and its
+// device address on device dev (looked up once per device).
+extern const uint32_t kScratchWords;
+hipError_t scratch_on_device(int dev, uint32_t **ptr);
 
 int set_error(int code, const std::string &msg);
 
@@ -136,6 +140,7 @@ class Arena {
     std::vector<NetSlot> slots;
     std::vector<NetRec> recs;
     hipStream_t stream = nullptr;
+    int device = 0;            // the device current at creation
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_run_ms = 0.0;
     uint32_t stim_cols = 0, stim_cap = 0;

@@ -28,6 +28,9 @@
 // The int ring (sim/src/kernels.rs:57-66) sums integers, which commute, so
 // every shape gives the CPU raster bit for bit whatever the atomic order.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
+
 #include "device.h"
 #include "layout.h"
 
@@ -781,6 +784,40 @@ __global__ void k_count_rows(const uint64_t *bits, uint32_t R, uint32_t words, u
         row = row == 0u ? R - 1u : row - 1u;
     }
     if (j < n) out[j] = c;
+}
+
+// Count scratch for callers that cannot allocate device memory
+// (nh_spike_counts_noalloc): a file-scope device array, 16 MiB, of which
+// every GPU the library runs on holds its own copy until the process ends.
+// On the host `scratch` is a stand-in, not a device address: host code uses
+// only the address scratch_on_device() returns, looked up with
+// hipGetSymbolAddress once per device with that device current.
+constexpr uint32_t kScratchWordsC = 1u << 22;
+static __device__ __align__(256) uint32_t scratch[kScratchWordsC];
+extern const uint32_t kScratchWords = kScratchWordsC;
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint32_t *g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_on_device(int dev, uint32_t **ptr) {
+    *ptr = nullptr;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return e;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = (uint32_t *)p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 void launch_count_rows(const uint64_t *bits, uint32_t R, uint32_t words, uint32_t base, uint32_t n,

@@ -278,6 +278,7 @@ int build_arena(nh_server *s) {
         r.active = 0;
     }
     ar.slots.resize(c.max_requests);
+    HIPCHK(hipGetDevice(&ar.device));
     HIPCHK(hipStreamCreateWithFlags(&ar.stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ar.ev0));
     HIPCHK(hipEventCreate(&ar.ev1));

@@ -455,6 +455,35 @@ int nh_spike_counts(nh_sim *s, uint64_t rows, uint32_t *out, uint32_t n) {
     });
 }
 
+// nh_spike_counts without device allocation: the window is counted into the
+// library's file-scope scratch (kScratchWords counts per device, shared by
+// every call on that device: calls that overlap on one GPU race).  Every
+// count the caller receives is written by the counting kernel of this call,
+// so nothing from an earlier call needs clearing.
+int nh_spike_counts_noalloc(nh_sim *s, uint64_t rows, uint32_t *out, uint32_t n) {
+    return nh::guarded([&]() -> int {
+        CHECK_SIM(s);
+        if (!out) return set_error(NH_ERR_ARG, "null output");
+        if (n < s->n) return set_error(NH_ERR_ARG, "output too small");
+        if (s->n > nh::kScratchWords)
+            return set_error(NH_ERR_CAPACITY, "spike_counts_noalloc: " + std::to_string(s->n) +
+                                                  " neurons exceed the scratch of " +
+                                                  std::to_string(nh::kScratchWords) + " counts");
+        std::memset(out, 0, (size_t)s->n * 4);
+        const uint64_t t = s->ar.t;
+        if (!s->cfg.record_spikes || t == 0 || s->n == 0 || rows == 0) return NH_OK;
+        if (rows >= t) return s->ar.read_pn(0, s->n, out);
+        const uint32_t R = s->ar.bits_rows;
+        if (rows > R)
+            return set_error(NH_ERR_WINDOW, "spike_counts window " + std::to_string(rows) +
+                                                " exceeds the recorded " + std::to_string(R) + " steps");
+        uint32_t *scr = nullptr;
+        const hipError_t e = nh::scratch_on_device(s->ar.device, &scr);
+        if (e != hipSuccess) return set_error(NH_ERR_HIP, std::string("count scratch: ") + hipGetErrorString(e));
+        return s->ar.count_rows(0, s->n, rows, scr, out);
+    });
+}
+
 int nh_set_input_constant(nh_sim *s, uint32_t input, float value) {
     CHECK_SIM(s);
     if (input >= s->n_inputs) return set_error(NH_ERR_ARG, "input out of range");

@@ -164,10 +164,13 @@ class Sim:
         L.check(L.lib().nh_last_spikes(self._h, L.ptr(out), self.n, C.byref(k)))
         return out[: k.value]
 
-    def spike_counts(self, rows):
+    def spike_counts(self, rows, no_alloc=False):
+        """Backend::spike_counts; no_alloc counts into the library's
+        scratch instead of a device buffer of the sim's own."""
         out = np.zeros(max(self.n, 1), np.uint32)
         rows = min(int(rows), (1 << 64) - 1)
-        rc = L.lib().nh_spike_counts(self._h, rows, L.ptr(out), self.n)
+        fn = L.lib().nh_spike_counts_noalloc if no_alloc else L.lib().nh_spike_counts
+        rc = fn(self._h, rows, L.ptr(out), self.n)
         if rc == L.NH_ERR_WINDOW:
             raise ValueError(L.lib().nh_last_error().decode())
         L.check(rc)
