@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
 B_NEURON = 33                   # algorithmic bytes per neuron-step
 B_EVENT = 17                    # algorithmic bytes per synaptic event
+PERSIST_MAX = 2048              # steps per persistent launch at most (library default)
 METRIC = "synaptic events/s & sim-ms/s, 100k Izh; batched/served vs plain, same raster"
 
 
@@ -422,11 +423,13 @@ def main():
         fused = aux_n == 0
     # steps per launch of the dominant kernel: the paired kernel runs two
     # (every launch of the timed region is a pair: chunks and the timed
-    # region hold even step counts)
-    spl = 2 if shape == 2 else 1
+    # region hold even step counts); the persistent kernel runs the timed
+    # region in launches of up to PERSIST_MAX steps
+    spl = 2 if shape == 2 else (min(args.steps, PERSIST_MAX) if shape == 3 else 1)
     if fused:
-        kname = ("k_pair (two steps per launch: update + own-spike push, delay-1 arrivals pulled)"
-                 if shape == 2 else "k_step (fused: update + own-spike push)")
+        kname = {2: "k_pair (two steps per launch: update + own-spike push, delay-1 arrivals pulled)",
+                 3: "k_run (persistent: all steps of a run in one launch, delay-1 arrivals pulled)"}.get(
+                     shape, "k_step (fused: update + own-spike push)")
         kbytes = spl * (args.n * B_NEURON + ev_per_step * B_EVENT)
         kavg = gpu_ms / (args.steps / spl)
     elif main_ms >= aux_ms:
@@ -438,7 +441,7 @@ def main():
         kbytes = ev_per_step * B_EVENT
         kavg = aux_ms / max(aux_n, 1)
     achieved = kbytes / (kavg * 1e-3) / 1e9
-    tname = "paired" if shape == 2 else ("fused" if fused else "neuron")
+    tname = {2: "paired", 3: "persistent"}.get(shape, "fused" if fused else "neuron")
     traffic = load_traffic(f"traffic_{tname}_cube{args.n // 1000}k_{args.ring}.txt")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -485,7 +488,8 @@ def main():
         sim2.synchronize()
         w2 = time.perf_counter() - t1
         ev2 = sim2.counts_since_mark()[1]
-        spl2 = 2 if sim2.launch_shape() == 2 else 1
+        sh2 = sim2.launch_shape()
+        spl2 = 2 if sh2 == 2 else (min(args.steps, PERSIST_MAX) if sh2 == 3 else 1)
         kb2 = spl2 * (args.n * B_NEURON + ev2 / args.steps * B_EVENT)
         ka2 = sim2.last_run_ms() / (args.steps / spl2)
         other = {"ring": "int Q16.16" if kw2["int_weights"] else "float",

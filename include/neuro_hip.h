@@ -81,7 +81,15 @@ typedef struct {
                                   first step pulled by their targets' waves); used where
                                   the whole grid fits on the device at once and no
                                   neuron is watched and no synapse has its own tau,
-                                  otherwise runs as 1.  Auto = 2 where at most 256
+                                  otherwise runs as 1.  3 many steps per launch (one
+                                  persistent kernel per chunk of nh_run_steps; the
+                                  delay-1 arrivals pulled, the rest gated on the
+                                  launch's own progress tags): one network without
+                                  stimulus inputs, watches or custom-tau synapses
+                                  whose grid fits on the device at once, otherwise
+                                  runs as 1; same results, never chosen by auto
+                                  (slower per step than 2 at every run length
+                                  measured).  Auto = 2 where at most 256
                                   delay-1 synapses end in any 64-neuron group, else 1 */
     uint32_t graph_steps;      /* steps per captured graph (0 = 64) */
     uint32_t block;            /* workgroup size of the step kernels, 64..512 (0 = 256) */
@@ -216,7 +224,8 @@ int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *mai
  * the trials goes to report (may be NULL). */
 int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_cap);
 /* Launch shape the next nh_run_steps uses (addition): 0 two kernels per
- * step, 1 one fused kernel per step, 2 two steps per launch. */
+ * step, 1 one fused kernel per step, 2 two steps per launch, 3 many steps
+ * per launch. */
 int nh_launch_shape(nh_sim *s, int32_t *shape);
 /* Test hook for the failure path of the two-steps-per-launch shape (no
  * counterpart in the reference): spin_limit = polls before a waiting wave

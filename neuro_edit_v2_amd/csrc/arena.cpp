@@ -368,12 +368,12 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         h_deg_[i] = (src_ptr[i + 1] - src_ptr[i]) + (n_x ? xs_hi[i] - xs_lo[i] : 0);
     // one kernel per step (own-spike push) unless the two-kernel or the
     // paired shape is asked for
-    fused = tune.fused == 0 ? 0 : (tune.fused == 2 ? 2 : 1);
+    fused = tune.fused == 0 ? 0 : (tune.fused == 2 || tune.fused == 3 ? tune.fused : 1);
     // Delay-1 in-tables for the paired shape (asked for, or left to the
     // autotuner on an arena small enough to be co-resident).  The ring then
     // gets one slot more than the deepest delay needs, so the pushes of a
     // paired launch never reach the two slots it reads.
-    pair_tables = (tune.fused == 2 || (tune.fused == -1 && n_total <= (1u << 20))) && n_x == 0 &&
+    pair_tables = (tune.fused == 2 || tune.fused == 3 || (tune.fused == -1 && n_total <= (1u << 20))) && n_x == 0 &&
                   n_total > 0 && n_total < (1u << 26);
     std::vector<uint32_t> pl_ptr, pl_ent;
     std::vector<int32_t> pl_wq;
@@ -555,6 +555,10 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         else TRY(upload(d_pl_w, pl_wf));
         TRY(alloc(d_pub, (size_t)words * 16));
         TRY(alloc(d_pair_abort, 4));
+        HIPCHK(h_pair_abort.alloc());
+        TRY(alloc(d_pub3, (size_t)words * 16 * kPubDepth));
+        TRY(alloc(d_gctr, 4 * 64 * 4));
+        TRY(alloc(d_done, 4 * 64 * 4));
     }
     TRY(alloc(d_totals, 8 * 8));
     if (n_x) {
@@ -690,8 +694,14 @@ StepArgs Arena::args() const {
     a.pl_w = d_pl_w.p;
     a.pub = d_pub.as<uint64_t>();
     a.pair_abort = d_pair_abort.as<uint32_t>();
+    a.pair_abort_host = h_pair_abort.dev;
     a.pair_spins = pair_spins ? pair_spins : kPairSpinsDefault;
     a.pair_mute = pair_mute;
+    a.pub3 = d_pub3.as<uint64_t>();
+    a.gctr = d_gctr.as<uint32_t>();
+    a.done = d_done.as<uint32_t>();
+    a.n_groups = run_groups;
+    a.run_steps = 1;
     a.xrec = d_xrec.as<float2>();
     a.xrec_depth = xrec_depth;
     return a;
@@ -771,6 +781,13 @@ int Arena::check_args(const StepArgs &a) const {
         TRY(need(a.watch, "watch"));
         TRY(need(a.xrec, "xrec"));
     }
+    if (fused == 3 && pair_tables) {
+        TRY(need(a.pub3, "pub3"));
+        TRY(need(a.gctr, "gctr"));
+        TRY(need(a.done, "done"));
+        if (a.n_groups == 0 || a.n_groups > 64 || a.n_groups > neuron_blocks())
+            return set_error(NH_ERR_ARG, "bad workgroup group count");
+    }
     if (a.pl_ptr) {
         TRY(need(a.pl_hi, "pl_hi"));
         TRY(need(a.pl_ent, "pl_ent"));
@@ -781,6 +798,24 @@ int Arena::check_args(const StepArgs &a) const {
     if (a.block % 64 || a.block == 0 || a.block > (uint32_t)kMaxBlock)
         return set_error(NH_ERR_ARG, "bad workgroup size");
     return NH_OK;
+}
+
+// The persistent kernel waits on every workgroup of its launch, so it runs
+// only when the whole grid fits on the device at once (at most half of the
+// waves the device holds for it, as for the paired kernel) and for the lean
+// case it covers: one network, no stimulus, no custom-tau synapses, no
+// window counters, no watched neurons.
+bool Arena::run_usable() {
+    if (fused != 3 || !pair_tables || !lean() || n_watch || n_x || words == 0 || !d_pub3.p) return false;
+    if (run_cap_block != block) {
+        run_cap_waves = run_resident_waves(ring_int, pmode, mmode, packed != 0, block);
+        run_cap_block = block;
+    }
+    const uint64_t need = (uint64_t)neuron_blocks() * (block / 64) * 2;
+    if (need > run_cap_waves) return false;
+    // groups of workgroups for the done tags: at most 64 (one poll lane each)
+    run_groups = std::min<uint32_t>(32u, neuron_blocks());
+    return true;
 }
 
 // The paired kernel waits on other waves of its own launch, so it runs only
@@ -808,9 +843,10 @@ bool Arena::pair_usable() {
 int Arena::check_pair_abort() {
     if (!pair_launched || !d_pair_abort.p) return NH_OK;
     pair_launched = false;
-    uint32_t f = 0;
-    HIPCHK(hipMemcpy(&f, d_pair_abort.p, 4, hipMemcpyDeviceToHost));
-    if (!f) return NH_OK;
+    // the host mirror (mapped memory, written by the wave that gave up) is
+    // read directly: no copy on the common path
+    if (!h_pair_abort.get()) return NH_OK;
+    h_pair_abort.clear();
     HIPCHK(hipMemsetAsync(d_pair_abort.p, 0, 4, stream));
     HIPCHK(hipStreamSynchronize(stream));
     return set_error(NH_ERR_HIP, "paired-step launch: a wave's wait for its delay-1 sources ran out "
@@ -842,6 +878,20 @@ int Arena::launch_chunk(uint64_t t0, uint32_t k) {
     return NH_OK;
 }
 
+// One persistent launch of k steps from arena step t0 (run_usable() true),
+// then the chunk advance of the device step counter.
+int Arena::launch_persistent(uint64_t t0, uint32_t k) {
+    StepArgs a = args();
+    a.k = 0;
+    a.slot = (uint32_t)(t0 % D);
+    a.run_steps = k;
+    TRY(check_args(a));
+    launch_run(a, neuron_blocks(), ring_int, pmode, mmode, packed != 0, stream);
+    launch_advance(d_t.as<uint64_t>(), k, stream);
+    HIPCHK(hipGetLastError());
+    return NH_OK;
+}
+
 int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,
                    uint64_t &aux_n) {
     main_ms = aux_ms = 0.0;
@@ -866,6 +916,27 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
             k = std::min<uint64_t>(k, room);
         }
         if ((rc = upload_stim(t, (uint32_t)k))) break;
+        if (run_usable()) {
+            // one persistent launch of the k steps: one main launch
+            (void)hipEventRecord(ev[0], stream);
+            if ((rc = launch_persistent(t, (uint32_t)k))) break;
+            (void)hipEventRecord(ev[1], stream);
+            pair_launched = true;
+            hipError_t e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) {
+                rc = set_error(NH_ERR_HIP, std::string("profile: ") + hipGetErrorString(e));
+                break;
+            }
+            if ((rc = check_pair_abort())) break;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
+                main_ms += ms;
+                ++main_n;
+            }
+            t += k;
+            n_steps -= k;
+            continue;
+        }
         StepArgs a = args();
         if ((rc = check_args(a))) break;
         // paired launches count as one main launch of two steps
@@ -999,6 +1070,29 @@ int Arena::run(uint64_t n_steps) {
         }
         if (graph_steps > stim_cap) use_graph = 0;
     }
+    if (run_usable()) {
+        // persistent kernel: one launch per chunk of steps, no graph
+        if (n_steps == 0) return NH_OK;
+        HIPCHK(hipEventRecord(ev0, stream));
+        while (n_steps) {
+            uint64_t k = std::min<uint64_t>(n_steps, run_max_steps);
+            if (rast_steps) {
+                const uint64_t room = rast_steps - (t - rast_step0);
+                if (room == 0) {
+                    TRY(drain_raster());
+                    continue;
+                }
+                k = std::min<uint64_t>(k, room);
+            }
+            TRY(launch_persistent(t, (uint32_t)k));
+            pair_launched = true;
+            t += k;
+            n_steps -= k;
+        }
+        HIPCHK(hipEventRecord(ev1, stream));
+        timing_valid = true;
+        return NH_OK;
+    }
     if (use_graph && graph_dirty) TRY(capture_graph());
     if (n_steps == 0) return NH_OK;  // a zero-step run only prepares the graph
     HIPCHK(hipEventRecord(ev0, stream));
@@ -1057,9 +1151,15 @@ int Arena::reset() {
     HIPCHK(hipMemsetAsync(d_net_spikes.p, 0, d_net_spikes.bytes, stream));
     HIPCHK(hipMemsetAsync(d_rast_ctl.p, 0, d_rast_ctl.bytes, stream));
     HIPCHK(hipMemsetAsync(d_t.p, 0, 8, stream));
+    if (d_pub3.p) {
+        HIPCHK(hipMemsetAsync(d_pub3.p, 0, d_pub3.bytes, stream));
+        HIPCHK(hipMemsetAsync(d_gctr.p, 0, d_gctr.bytes, stream));
+        HIPCHK(hipMemsetAsync(d_done.p, 0, d_done.bytes, stream));
+    }
     if (d_pub.p) {
         HIPCHK(hipMemsetAsync(d_pub.p, 0, d_pub.bytes, stream));
         HIPCHK(hipMemsetAsync(d_pair_abort.p, 0, d_pair_abort.bytes, stream));
+        h_pair_abort.clear();
     }
     HIPCHK(hipStreamSynchronize(stream));
     pair_launched = false;
@@ -1072,7 +1172,7 @@ int Arena::reset() {
 }
 
 static std::string shape_name(int f) {
-    return f == 2 ? "paired" : (f ? "fused" : "two-kernel");
+    return f == 3 ? "persistent" : (f == 2 ? "paired" : (f ? "fused" : "two-kernel"));
 }
 
 int Arena::autotune(uint32_t trial_steps, std::string &report) {
@@ -1084,17 +1184,18 @@ int Arena::autotune(uint32_t trial_steps, std::string &report) {
         double ms;
     };
     std::vector<Cand> cands;
-    for (int f : {1, 0, 2})
+    for (int f : {1, 0, 2, 3})
         for (uint32_t b : {64u, 128u, 256u, 512u})
             for (uint32_t g : {64u, 256u}) {
                 if (!use_graph && g != 64u) continue;
-                if (f == 2) {
-                    // only where a paired launch would actually run
+                if (f == 3 && g != 64u) continue;   // no graph: one launch per chunk
+                if (f >= 2) {
+                    // only where a paired / persistent launch would actually run
                     const int keep_f = fused;
                     const uint32_t keep_b = block;
-                    fused = 2;
+                    fused = f;
                     block = b;
-                    const bool ok = pair_usable();
+                    const bool ok = f == 2 ? pair_usable() : run_usable();
                     fused = keep_f;
                     block = keep_b;
                     if (!ok) continue;

@@ -26,6 +26,8 @@ void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mm
                  bool lean, bool w8, hipStream_t st);
 uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool lean, bool w8,
                              uint32_t block);
+void launch_run(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed, hipStream_t st);
+uint32_t run_resident_waves(int ring, int pmode, int mmode, bool packed, uint32_t block);
 void launch_count_rows(const uint64_t *bits, uint32_t R, uint32_t words, uint32_t base, uint32_t n,
                        uint64_t t, uint32_t rows, uint32_t *out, hipStream_t st);
 // Words of the file-scope count scratch (nh_spike_counts_noalloc), and its
@@ -68,6 +70,35 @@ struct DBuf {
     template <class T> T *as() const { return (T *)p; }
 };
 
+// A 32-bit word of pinned host memory mapped into the device's address
+// space: device code writes it rarely (a wait that gave up), and the host
+// reads it after a synchronisation without a copy.
+struct PinnedWord {
+    uint32_t *host = nullptr;
+    uint32_t *dev = nullptr;
+    PinnedWord() = default;
+    PinnedWord(const PinnedWord &) = delete;
+    PinnedWord &operator=(const PinnedWord &) = delete;
+    ~PinnedWord() {
+        if (host) (void)hipHostFree(host);
+    }
+    hipError_t alloc() {
+        if (host) return hipSuccess;
+        hipError_t e = hipHostMalloc((void **)&host, 4, hipHostMallocMapped | hipHostMallocCoherent);
+        if (e != hipSuccess) {
+            host = nullptr;
+            return e;
+        }
+        *(volatile uint32_t *)host = 0u;
+        e = hipHostGetDevicePointer((void **)&dev, host, 0);
+        return e;
+    }
+    uint32_t get() const { return host ? *(volatile uint32_t *)host : 0u; }
+    void clear() {
+        if (host) *(volatile uint32_t *)host = 0u;
+    }
+};
+
 // One packed network as the arena sees it.
 struct NetSlot {
     uint32_t base = 0, n = 0;
@@ -76,7 +107,8 @@ struct NetSlot {
 };
 
 struct Tuning {
-    int fused = -1;          // -1 auto, 0 two-phase, 1 fused, 2 paired (two steps per launch)
+    int fused = -1;          // -1 auto, 0 two-phase, 1 fused, 2 paired (two steps per launch),
+                             // 3 persistent (many steps per launch)
     int use_graph = -1;      // -1 auto
     uint32_t graph_steps = 64;
     uint32_t scatter_blocks = 0;  // 0 auto
@@ -175,6 +207,7 @@ class Arena {
     // wait-abort flag; built when the shape may be used (the ring then has
     // one slot more than the deepest delay needs)
     DBuf d_pl_ptr, d_pl_hi, d_pl_ent, d_pl_w, d_pub, d_pair_abort;  // d_pl_hi: served arena
+    PinnedWord h_pair_abort;   // host mirror of d_pair_abort
     bool pair_tables = false;
     bool pair_launched = false;          // paired launches since the last abort check
     uint32_t pair_cap_block = 0, pair_cap_waves = 0, pair_cap8_waves = 0;
@@ -193,6 +226,15 @@ class Arena {
     // slower per step at 1.6k waves, 0.17 us faster at 4.7k, 0.54 us at 15.6k)
     bool lean_step() const { return lean() && (uint64_t)neuron_blocks() * (block / 64) >= 3072; }
     int check_pair_abort();
+    // persistent kernel (many steps per launch): published words kPubDepth
+    // steps deep, per-residue group counters and done tags; usable when
+    // fused == 3, the arena is lean, has the delay-1 tables and its grid is
+    // co-resident
+    DBuf d_pub3, d_gctr, d_done;
+    uint32_t run_groups = 1;
+    uint32_t run_cap_block = 0, run_cap_waves = 0;
+    uint32_t run_max_steps = 2048;      // steps per persistent launch at most
+    bool run_usable();
     std::vector<uint32_t> h_deg_;      // total out-degree per arena neuron
     uint32_t n_win = 0;
     bool dynamic = false;      // served arena: slots come and go
@@ -208,6 +250,7 @@ class Arena {
 
     StepArgs args() const;
     int launch_chunk(uint64_t t0, uint32_t k);
+    int launch_persistent(uint64_t t0, uint32_t k);
     int check_args(const StepArgs &a) const;
     // direct launches with an event pair around every step kernel
     int profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,

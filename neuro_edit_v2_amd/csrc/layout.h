@@ -16,6 +16,7 @@ constexpr int kMaxBlock = 512;       // largest workgroup the step kernels accep
 constexpr int kMaxParams = 10;       // AdEx row width
 constexpr uint32_t kNoSpike = 0xFFFFFFFFu;
 constexpr uint32_t kPairSpinsDefault = 1u << 16;  // polls before a paired-step wave gives up
+constexpr uint32_t kPubDepth = 3;    // persistent kernel: steps of published spike words kept
 
 // Per-network scalars (one row per packed network).  All fields are read
 // with scalar loads (the index is wave-uniform).
@@ -142,8 +143,20 @@ struct StepArgs {
     const void *pl_w;
     uint64_t *pub;
     uint32_t *pair_abort;
+    uint32_t *pair_abort_host;   // the same flag in mapped host memory (read by the host without a copy)
     uint32_t pair_spins;         // polls before a wave gives up waiting
     uint32_t pair_mute;          // test only: 1 = wave 0 never publishes
+    // persistent kernel (many steps per launch): steps of the launch; the
+    // spike words it publishes, kPubDepth steps deep (pub3[(t % 3)][2w +
+    // half], tag t + 1); per step residue r = t % 4 and group g of
+    // workgroups (workgroup b is in group b % n_groups), the group's arrival
+    // counter gctr[r][g] and its done tag done[r][g] = t + 1, published once
+    // every workgroup of the group has finished step t (pushes complete)
+    uint32_t run_steps;
+    uint64_t *pub3;
+    uint32_t *gctr;
+    uint32_t *done;
+    uint32_t n_groups;           // 1..64
     // watched neurons (record ring of state rows)
     const uint32_t *watch;
     uint32_t n_watch;

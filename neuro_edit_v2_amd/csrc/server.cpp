@@ -330,6 +330,7 @@ int build_arena(nh_server *s) {
         TRY(A(ar.d_pl_w, c.cap_synapses * 4));
         TRY(A(ar.d_pub, (uint64_t)ar.words * 16));
         TRY(A(ar.d_pair_abort, 4));
+        HIPCHK(ar.h_pair_abort.alloc());
         ar.pair_tables = true;
         ar.fused = 2;
     }

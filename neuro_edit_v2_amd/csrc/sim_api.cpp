@@ -220,7 +220,7 @@ int nh_test_pair_wait(nh_sim *s, uint32_t spin_limit, uint32_t mute_wave0) {
 int nh_launch_shape(nh_sim *s, int32_t *shape) {
     CHECK_SIM(s);
     if (!shape) return set_error(NH_ERR_ARG, "null output");
-    *shape = s->ar.pair_usable() ? 2 : (s->ar.fused ? 1 : 0);
+    *shape = s->ar.run_usable() ? 3 : (s->ar.pair_usable() ? 2 : (s->ar.fused ? 1 : 0));
     return NH_OK;
 }
 
