@@ -201,3 +201,59 @@ def test_ring_above_2gb_uses_global_atomics_bit_exact():
     assert rc.sum() > 10000
     assert np.array_equal(cnt, rc) and np.array_equal(ids, ri)
     assert np.array_equal(v, ref.v().astype(np.float32))
+
+
+def test_batched_workload_full_size_float_ring():
+    """The 256-network batched workload with the float ring: four networks
+    (two Izhikevich, two AdEx) against the oracle -- v and u of every neuron
+    within 1e-4 relative (denominator max(|x|, 1)) after 200 steps, and over
+    the 1,000 steps per-neuron spike counts within 2 % mean absolute
+    deviation and totals within 2 %.  Every network's total within 2 % of
+    its own run alone."""
+    from neuro_edit_v2_amd import workloads as W
+    from neuro_edit_v2_amd.batch import Batch
+    steps, dt = 1000, 0.5
+    nets, series = W.batch_nets(256, dt=dt, steps=steps)
+    cfgs = [SimConfig(dt_ms=dt, synapse_tau_ms=5.0, seed=k + 1, int_weights=False,
+                      raster_steps=steps) for k in range(len(nets))]
+    print("case: batched 256 nets, float ring", flush=True)
+    b = Batch(nets, cfgs)
+    for k in range(len(nets)):
+        for q in range(8):
+            b.play_input(k, q, series[k][:, q])
+    picks = (0, 1, 2, 3)
+    refs = {}
+    for k in picks:
+        ref = O.OracleSim(nets[k], O.default_cfg(dt_ms=dt, synapse_tau_ms=5.0, seed=k + 1, exp_mode=1))
+        for q in range(8):
+            ref.play_input(q, series[k][:, q])
+        refs[k] = ref
+    b.run_steps(200)
+    worst = 0.0
+    for k in picks:
+        refs[k].run_steps(200)
+        for slot in (0, 1):
+            g = b.state(k, slot).astype(np.float64)
+            r = refs[k].state(slot)
+            worst = max(worst, float((np.abs(g - r) / np.maximum(np.abs(r), 1.0)).max()))
+    print(f"200 steps: max rel v/u error {worst:.2e}")
+    assert worst < 1e-4
+    b.run_steps(steps - 200)
+    for k in range(len(nets)):
+        _, bc, bi = b.read_spikes(k)
+        g = np.bincount(bi, minlength=int(nets[k]["n"])).astype(np.float64)
+        if k in picks:
+            refs[k].run_steps(steps - 200)
+            r = refs[k].pn_total().astype(np.float64)
+            mad = np.abs(g - r).mean() / max(r.mean(), 1e-9)
+            print(f"network {k}: spikes gpu {g.sum():.0f} oracle {r.sum():.0f}, count MAD {mad:.4f}")
+            assert r.sum() > 0 and abs(g.sum() - r.sum()) <= 0.02 * r.sum()
+            assert mad < 0.02
+        if k < 32:
+            s = Sim(nets[k], cfgs[k])
+            for q in range(8):
+                s.play_input(q, series[k][:, q])
+            s.run_steps(steps)
+            a = s.spikes_total()
+            s.close()
+            assert abs(a - g.sum()) <= 0.02 * max(a, 1) + 2, f"network {k}"
