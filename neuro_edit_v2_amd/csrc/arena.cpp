@@ -169,6 +169,21 @@ int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay) {
         if (d.tau && !(fin(d.tau[j]) && d.tau[j] >= 0.0f)) return set_error(NH_ERR_ARG, "synapse tau must be finite and >= 0");
         maxd = std::max<uint32_t>(maxd, d.delay[j]);
     }
+    // Int ring: a slot of neuron i sums at most one arrival per in-synapse
+    // (a synapse of delay d delivers into slot t only from a spike at t - d),
+    // so the slot cannot wrap if the in-synapses' quantised magnitudes sum
+    // below 2^31.  Refused otherwise: the CUDA int ring and the oracle would
+    // both wrap silently.
+    if (c.int_weights && E) {
+        std::vector<int64_t> fan(d.n, 0);
+        for (uint64_t j = 0; j < E; ++j)
+            fan[d.tgt[j]] += std::llabs((long long)quantize(d.w[j], c.int_scale));
+        for (uint32_t i = 0; i < d.n; ++i)
+            if (fan[i] > 2147483647LL)
+                return set_error(NH_ERR_NET, "neuron " + std::to_string(i) +
+                                                 ": its in-synapses' weights x int_scale sum past the int32 "
+                                                 "ring (a slot could overflow); lower int_scale");
+    }
     if (d.n_inputs && d.stim_ptr) {
         if (!d.stim_input && d.stim_ptr[d.n]) return set_error(NH_ERR_ARG, "stimulus arrays missing");
         if (d.stim_ptr[0] != 0) return set_error(NH_ERR_ARG, "stim_ptr[0] must be 0");

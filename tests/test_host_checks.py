@@ -100,3 +100,21 @@ def test_create_checks_before_touching_the_device(net):
     with pytest.raises(L.NHError) as e:
         Sim(net)
     assert e.value.code == L.NH_ERR_NO_DEVICE
+
+
+def test_int_ring_fan_in_overflow_refused():
+    """Int ring: a slot of a neuron sums at most one arrival per in-synapse,
+    so the check refuses a network whose in-synapse weights x int_scale sum
+    past 2^31 - 1 for some neuron (the ring would wrap silently), accepts it
+    at a scale where the sum fits, and accepts it with the float ring."""
+    n = N.grid_net(5, n_inputs=2, per_input=4, seed=3)
+    w = n["w"].copy()
+    hub = int(np.bincount(n["tgt"]).argmax())
+    into = np.flatnonzero(n["tgt"] == hub)
+    assert len(into) >= 3
+    w[into] = np.sign(w[into]) * 15000.0     # each |w| * 65536 < 2^31, their sum is not
+    n["w"] = w
+    rc, msg = check(n, int_weights=True, int_scale=65536.0)
+    assert rc == L.NH_ERR_NET and "sum past the int32" in msg and f"neuron {hub}" in msg
+    assert check(n, int_weights=True, int_scale=4096.0)[0] == L.NH_OK
+    assert check(n, int_weights=False)[0] == L.NH_OK
