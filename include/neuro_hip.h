@@ -93,6 +93,10 @@ typedef struct {
                                   delay-1 synapses end in any 64-neuron group, else 1 */
     uint32_t graph_steps;      /* steps per captured graph (0 = 64) */
     uint32_t block;            /* workgroup size of the step kernels, 64..512 (0 = 256) */
+    uint32_t lds_bins;         /* one-step shape (1): push through a workgroup table of
+                                  this many LDS bins (a power of two, 256..8192), one
+                                  HBM atomic per distinct ring word; 0 = each spiking
+                                  wave pushes its own rows */
 } nh_config;
 
 /* SimConfig::default() values plus neutral additions. */

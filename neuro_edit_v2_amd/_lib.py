@@ -36,7 +36,7 @@ class Config(C.Structure):
         ("spike_record_steps", C.c_uint32), ("int_weights", C.c_int32),
         ("int_scale", C.c_float), ("raster_steps", C.c_uint32),
         ("exec_mode", C.c_int32), ("step_shape", C.c_int32), ("graph_steps", C.c_uint32),
-        ("block", C.c_uint32),
+        ("block", C.c_uint32), ("lds_bins", C.c_uint32),
     ]
 
 

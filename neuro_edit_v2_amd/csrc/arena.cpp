@@ -615,6 +615,14 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
             return set_error(NH_ERR_ARG, "block must be a multiple of 64 up to 512");
         block = tune.block;
     }
+    if (tune.lds_bins) {
+        const uint32_t b = tune.lds_bins;
+        if (b < 256u || b > 8192u || (b & (b - 1u)))
+            return set_error(NH_ERR_ARG, "lds_bins must be 0 or a power of two in 256..8192");
+        if ((uint64_t)D * n_total >= 0xFFFFFFFFull)
+            return set_error(NH_ERR_ARG, "lds_bins: the ring is too large for 32-bit bin keys");
+        lds_bins = b;
+    }
     use_graph = tune.use_graph >= 0 ? tune.use_graph : 1;
     graph_steps_req = tune.graph_steps ? tune.graph_steps : 64;
     graph_steps = 0;
@@ -717,6 +725,7 @@ StepArgs Arena::args() const {
     a.done = d_done.as<uint32_t>();
     a.n_groups = run_groups;
     a.run_steps = 1;
+    a.bin_entries = binned() ? lds_bins : 0u;
     a.xrec = d_xrec.as<float2>();
     a.xrec_depth = xrec_depth;
     return a;
@@ -883,7 +892,8 @@ int Arena::launch_chunk(uint64_t t0, uint32_t k) {
             j += 2;
             continue;
         }
-        launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, lean_step(), stream);
+        if (binned()) launch_step_binned(a, nb, ring_int, pmode, mmode, packed != 0, stream);
+        else launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, lean_step(), stream);
         if (!fused) launch_scatter(a, sb, ring_int, packed != 0, stream);
         if (n_watch) launch_watch(a, stream);
         ++j;
@@ -969,7 +979,8 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
                 j += 2;
                 continue;
             }
-            launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, lean_step(), stream);
+            if (binned()) launch_step_binned(a, nb, ring_int, pmode, mmode, packed != 0, stream);
+            else launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, lean_step(), stream);
             (void)hipEventRecord(ev[4 * j + 1], stream);
             if (!fused) {
                 (void)hipEventRecord(ev[4 * j + 2], stream);
@@ -1195,34 +1206,42 @@ int Arena::autotune(uint32_t trial_steps, std::string &report) {
     if (n_total == 0 || dynamic) return NH_OK;
     struct Cand {
         int fused;
-        uint32_t block, graph;
+        uint32_t block, graph, bins;
         double ms;
     };
+    // bin tables for the one-step shape's LDS-binned push (0 = the
+    // wave-by-wave push); only where 32-bit bin keys cover the ring
+    std::vector<uint32_t> bin_opts = {0u};
+    if ((uint64_t)D * n_total < 0xFFFFFFFFull) bin_opts.insert(bin_opts.end(), {1024u, 4096u});
     std::vector<Cand> cands;
     for (int f : {1, 0, 2, 3})
         for (uint32_t b : {64u, 128u, 256u, 512u})
-            for (uint32_t g : {64u, 256u}) {
-                if (!use_graph && g != 64u) continue;
-                if (f == 3 && g != 64u) continue;   // no graph: one launch per chunk
-                if (f >= 2) {
-                    // only where a paired / persistent launch would actually run
-                    const int keep_f = fused;
-                    const uint32_t keep_b = block;
-                    fused = f;
-                    block = b;
-                    const bool ok = f == 2 ? pair_usable() : run_usable();
-                    fused = keep_f;
-                    block = keep_b;
-                    if (!ok) continue;
+            for (uint32_t g : {64u, 256u})
+                for (uint32_t bins : bin_opts) {
+                    if (!use_graph && g != 64u) continue;
+                    if (f == 3 && g != 64u) continue;   // no graph: one launch per chunk
+                    if (f != 1 && bins) continue;       // bins belong to the one-step shape
+                    if (bins && b < 256u) continue;     // a bin table per small workgroup: LDS-bound
+                    if (f >= 2) {
+                        // only where a paired / persistent launch would actually run
+                        const int keep_f = fused;
+                        const uint32_t keep_b = block;
+                        fused = f;
+                        block = b;
+                        const bool ok = f == 2 ? pair_usable() : run_usable();
+                        fused = keep_f;
+                        block = keep_b;
+                        if (!ok) continue;
+                    }
+                    cands.push_back({f, b, g, bins, 0.0});
                 }
-                cands.push_back({f, b, g, 0.0});
-            }
-    Cand best{fused, block, graph_steps_req, 1e30};
+    Cand best{fused, block, graph_steps_req, lds_bins, 1e30};
     report.clear();
     for (Cand &c : cands) {
         fused = c.fused;
         block = c.block;
         graph_steps_req = c.graph;
+        lds_bins = c.bins;
         graph_dirty = true;
         TRY(reset());
         TRY(run(0));                       // capture outside the timed trial
@@ -1231,18 +1250,20 @@ int Arena::autotune(uint32_t trial_steps, std::string &report) {
         TRY(run(trial_steps));
         TRY(sync());
         c.ms = last_run_ms;
-        report += shape_name(c.fused) + std::string(" block ") +
-                  std::to_string(c.block) + " graph " + std::to_string(graph_steps) + ": " +
-                  std::to_string(c.ms * 1e3 / trial_steps) + " us/step\n";
+        report += shape_name(c.fused) + std::string(" block ") + std::to_string(c.block) + " graph " +
+                  std::to_string(graph_steps) + (c.bins ? " bins " + std::to_string(c.bins) : std::string()) +
+                  ": " + std::to_string(c.ms * 1e3 / trial_steps) + " us/step\n";
         if (c.ms < best.ms) best = c;
     }
     fused = best.fused;
     block = best.block;
     graph_steps_req = best.graph;
+    lds_bins = best.bins;
     graph_dirty = true;
     TRY(reset());
-    report += "chosen: " + shape_name(best.fused) + " block " +
-              std::to_string(best.block) + " graph " + std::to_string(best.graph) + "\n";
+    report += "chosen: " + shape_name(best.fused) + " block " + std::to_string(best.block) + " graph " +
+              std::to_string(best.graph) + (best.bins ? " bins " + std::to_string(best.bins) : std::string()) +
+              "\n";
     return NH_OK;
 }
 

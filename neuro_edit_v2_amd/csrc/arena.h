@@ -17,6 +17,8 @@ namespace nh {
 int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay);
 void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool push,
                  bool packed, bool lean, hipStream_t st);
+void launch_step_binned(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
+                        hipStream_t st);
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st);
 void launch_watch(const StepArgs &a, hipStream_t st);
 void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
@@ -113,6 +115,7 @@ struct Tuning {
     uint32_t graph_steps = 64;
     uint32_t scatter_blocks = 0;  // 0 auto
     uint32_t block = 0;           // 0 = kBlock
+    uint32_t lds_bins = 0;        // LDS-binned push (one-step shape): bins per workgroup, 0 = off
 };
 
 class Arena {
@@ -240,6 +243,8 @@ class Arena {
     bool dynamic = false;      // served arena: slots come and go
     uint32_t grid_limit = 0;   // served arena: launch only the occupied prefix
     uint32_t block = kBlock;   // workgroup size of the step kernels
+    uint32_t lds_bins = 0;     // LDS-binned push: bins per workgroup (0 = wave-by-wave push)
+    bool binned() const { return fused == 1 && lds_bins != 0; }
     // Time each (shape, workgroup, graph chunk) candidate over trial_steps
     // from the initial state, keep the fastest, then reset (t must be 0).
     int autotune(uint32_t trial_steps, std::string &report);

@@ -101,6 +101,7 @@ int nh_batch_create(uint32_t n_nets, const nh_net_desc *nets, const nh_config *c
         tune.use_graph = cfgs[0].exec_mode == 0 ? -1 : (cfgs[0].exec_mode == 2 ? 1 : 0);
         tune.graph_steps = cfgs[0].graph_steps ? cfgs[0].graph_steps : 64;
         tune.block = cfgs[0].block;
+        tune.lds_bins = cfgs[0].lds_bins;
         TRY(b->ar.build(n_nets, nets, cfgs, cfgs[0].raster_steps, tune));
         b->cfgs.assign(cfgs, cfgs + n_nets);
         b->series.resize(n_nets);

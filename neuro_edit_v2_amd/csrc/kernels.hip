@@ -558,6 +558,174 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- binned
+// The fused step with the push done by the whole workgroup through LDS
+// bins (a packed arena, not the lean single-network case).  After the
+// update each wave appends its spiking neurons' rows to a workgroup queue
+// in LDS (row start and offset in the concatenated list, reserved with one
+// 64-bit LDS atomic per wave, so offsets ascend with the queue index) and
+// does its own stores; after a barrier a workgroup with spikes clears its
+// table and every thread of the workgroup walks the concatenated list (entry -> owning row by binary search over
+// the queue), loads the synapse, and adds its weight into an LDS hash table
+// keyed by ring word (linear probing, at most kBinProbes slots; an entry
+// that finds no slot goes straight to HBM); after a second barrier the
+// table's words are added to the ring with one HBM atomic each.  Deliveries
+// of the workgroup that land on the same ring word (same target, same
+// arrival step) leave the workgroup as one atomic, and the row walk is
+// spread over the workgroup's lanes instead of the spiking wave's.  Int
+// ring: integer sums, so the raster stays bit-exact; float ring: the sum
+// order changes (as with atomic order anyway).
+constexpr uint32_t kBinEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kBinProbes = 8;
+
+template <int RING, bool PACKED>
+__device__ __forceinline__ void ring_add_word(const StepArgs &a, uint32_t word, typename Ring<RING>::T v) {
+    using RT = typename Ring<RING>::T;
+    if (a.ring_buf) Ring<RING>::add_buf(buffer_rsrc(a.ring, a.ring_bytes), word * 4u, v);
+    else Ring<RING>::add((RT *)a.ring + word, v);
+}
+
+template <int RING, int PMODE, int MMODE, bool PACKED>
+__global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
+    using RT = typename Ring<RING>::T;
+    extern __shared__ uint32_t smem[];
+    const uint32_t B = a.bin_entries;
+    uint32_t *bkey = smem;
+    RT *bval = (RT *)(smem + B);
+    uint32_t *q_lo = smem + 2u * B;               // blockDim entries
+    uint32_t *q_off = q_lo + blockDim.x;          // blockDim entries
+    unsigned long long *q_ctr = (unsigned long long *)(q_off + blockDim.x);  // spikes << 32 | entries
+    if (threadIdx.x == 0) *q_ctr = 0ull;
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    uint32_t blk = blockIdx.x;
+    if (!a.single_net && gridDim.x % kBands == 0u) blk = (blk % kBands) * (gridDim.x / kBands) + blk / kBands;
+    const uint32_t wave = blk * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
+    const bool live = wave < a.words;              // every thread reaches the barriers
+    const uint32_t i = wave * kWave + lane;
+    const uint32_t slot = a.slot;
+    uint64_t mask = 0;
+    if (live) {
+        NetRec nr;
+        uint32_t net = 0;
+        if (a.single_net) {
+            nr = a.net0;
+        } else {
+            net = uniform((uint32_t)a.wave_net[wave]);
+            nr = a.nets[net];
+        }
+        RT *ring = (RT *)a.ring;
+        const size_t ri = (size_t)slot * a.n_total + i;
+        const RT rv = ring[ri];
+        const float s_old = a.s[i];
+        float2 xs = a.x[i];
+        const float ext = a.ext ? a.ext[i] : 0.0f;
+        const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
+        uint32_t prow = 0;
+        if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
+        const uint64_t t = *a.t_dev + a.k;
+        const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
+        const bool stim_on = a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
+        const uint32_t local = i - nr.base;
+        const uint64_t tl = t - (uint64_t)nr.t0;
+        const bool act = local < nr.n && nr.active && (nr.t_len == 0 || tl < nr.t_len);
+        bool spiked = false;
+        float s_new = s_old;
+        if (act)
+            spiked = neuron_update<RING, PMODE, MMODE, false, false, false>(
+                a, nr, i, local, tl, slot, a.k, stim_on, prow, rv, ext, Noise{false, 0.0f}, Params{}, s_new, xs);
+        mask = __ballot(spiked);
+        if (mask) {
+            // queue this wave's spiking rows: lane offsets in the wave's
+            // concatenated list, then one reservation for the wave
+            const uint32_t len = spiked ? row_hi - row_lo : 0u;
+            uint32_t excl = 0, total = 0;
+            for (uint64_t m = mask; m;) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const uint32_t lb = __builtin_amdgcn_readlane(len, b);
+                if (b < lane) excl += lb;
+                total += lb;
+            }
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(q_ctr, ((unsigned long long)__popcll(mask) << 32) | total);
+            const uint32_t bq = uniform((uint32_t)(base >> 32)), be = uniform((uint32_t)base);
+            if (spiked) {
+                const uint32_t q = bq + wave_rank(mask);
+                q_lo[q] = row_lo;
+                q_off[q] = be + excl;
+            }
+            if (a.xt) {
+                for (uint64_t m = mask; m;) {
+                    const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                    m &= m - 1;
+                    push_custom<RING>(a, wave * kWave + b, slot, lane);
+                }
+            }
+        }
+        // this step's stores (the pushes below never touch slot t)
+        if (act) {
+            ring[ri] = (RT)0;
+            a.x[i] = xs;
+            a.s[i] = s_new;
+            if (spiked) spike_marks<false>(a, nr, i, t, tl);
+        }
+        if (lane == 0) spike_words(a, wave, t, rast_origin, mask);
+    }
+    __syncthreads();
+
+    // ---- the workgroup's deliveries through the bins
+    const uint64_t qc = *q_ctr;
+    const uint32_t Q = (uint32_t)(qc >> 32), T = (uint32_t)qc;
+    if (T == 0) return;            // no spike in the workgroup: nothing to bin (uniform)
+    // the table is cleared only in workgroups that use it
+    for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) {
+        bkey[k] = kBinEmpty;
+        bval[k] = (RT)0;
+    }
+    __syncthreads();
+    {
+        using W = RT;
+        const W *w = (const W *)a.w;
+        const uint32_t shift = 32u - (uint32_t)__builtin_ctz(B);
+        for (uint32_t e = threadIdx.x; e < T; e += blockDim.x) {
+            // owning row: the last queue entry whose offset is <= e
+            uint32_t lo = 0, hi = Q;
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (q_off[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t j = q_lo[lo] + (e - q_off[lo]);
+            const uint32_t sy = a.syn[j];
+            const W wv = w[j];
+            const uint32_t d16 = PACKED ? 0u : a.delay16[j];
+            const uint32_t word = (uint32_t)ring_index<PACKED>(a, sy, d16, slot);
+            uint32_t h = (word * 0x9E3779B1u) >> shift;
+            bool placed = false;
+            for (uint32_t p = 0; p < kBinProbes; ++p) {
+                const uint32_t prev = atomicCAS(&bkey[h], kBinEmpty, word);
+                if (prev == kBinEmpty || prev == word) {
+                    atomicAdd(&bval[h], wv);
+                    placed = true;
+                    break;
+                }
+                h = (h + 1u) & (B - 1u);
+            }
+            if (!placed) ring_add_word<RING, PACKED>(a, word, wv);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) {
+        const uint32_t word = bkey[k];
+        if (word != kBinEmpty) ring_add_word<RING, PACKED>(a, word, bval[k]);
+    }
+}
+
+// LDS bytes of k_step_binned with `entries` bins and workgroups of `block`.
+uint32_t binned_lds_bytes(uint32_t entries, uint32_t block) { return entries * 8u + block * 8u + 8u; }
+
 // ---------------------------------------------------------------- pair
 // Two steps per launch, t0 = t and t1 = t + 1.  Between the two steps the
 // only dependency that crosses waves is a delay-1 synapse: a spike at t0
@@ -1189,6 +1357,38 @@ static void launch_r(const StepArgs &a, uint32_t blocks, int pmode, int mmode, b
     case 1: launch_p<RING, 1>(a, blocks, mmode, push, packed, lean, st); break;
     default: launch_p<RING, 2>(a, blocks, mmode, push, packed, lean, st); break;
     }
+}
+
+template <int RING, int PMODE, int MMODE>
+static void go_binned(const StepArgs &a, uint32_t blocks, bool packed, hipStream_t st) {
+    if (!a.single_net && blocks >= 4u * kBands) blocks = (blocks + kBands - 1u) / kBands * kBands;
+    const uint32_t lds = binned_lds_bytes(a.bin_entries, a.block);
+    if (packed)
+        hipLaunchKernelGGL((k_step_binned<RING, PMODE, MMODE, true>), dim3(blocks), dim3(a.block), lds, st, a);
+    else
+        hipLaunchKernelGGL((k_step_binned<RING, PMODE, MMODE, false>), dim3(blocks), dim3(a.block), lds, st, a);
+}
+
+template <int RING>
+static void binned_r(const StepArgs &a, uint32_t blocks, int pmode, int mmode, bool packed, hipStream_t st) {
+#define NH_BIN_M(P)                                                                               \
+    switch (mmode) {                                                                              \
+    case 0: go_binned<RING, P, 0>(a, blocks, packed, st); break;                                  \
+    case 1: go_binned<RING, P, 1>(a, blocks, packed, st); break;                                  \
+    default: go_binned<RING, P, 2>(a, blocks, packed, st); break;                                 \
+    }
+    switch (pmode) {
+    case 0: NH_BIN_M(0) break;
+    case 1: NH_BIN_M(1) break;
+    default: NH_BIN_M(2) break;
+    }
+#undef NH_BIN_M
+}
+
+void launch_step_binned(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
+                        hipStream_t st) {
+    if (ring == 0) binned_r<0>(a, blocks, pmode, mmode, packed, st);
+    else binned_r<1>(a, blocks, pmode, mmode, packed, st);
 }
 
 void launch_step(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool push,

@@ -157,6 +157,9 @@ struct StepArgs {
     uint32_t *gctr;
     uint32_t *done;
     uint32_t n_groups;           // 1..64
+    // LDS-binned push (k_step_binned): entries of the workgroup's bin table
+    // (a power of two; 0 = the wave-by-wave push)
+    uint32_t bin_entries;
     // watched neurons (record ring of state rows)
     const uint32_t *watch;
     uint32_t n_watch;

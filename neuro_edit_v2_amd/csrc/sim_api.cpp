@@ -109,6 +109,7 @@ void nh_config_default(nh_config *c) {
     c->step_shape = -1;
     c->graph_steps = 0;
     c->block = 0;
+    c->lds_bins = 0;
 }
 
 int nh_available(void) {
@@ -142,6 +143,7 @@ int nh_create(const nh_net_desc *net, const nh_config *cfg, nh_sim **out) {
         tune.use_graph = cfg->exec_mode == 0 ? -1 : (cfg->exec_mode == 2 ? 1 : 0);
         tune.graph_steps = cfg->graph_steps ? cfg->graph_steps : 64;
         tune.block = cfg->block;
+        tune.lds_bins = cfg->lds_bins;
         TRY(s->ar.build(1, net, cfg, cfg->raster_steps, tune));
         s->constants.assign(s->n_inputs, 0.0f);
         s->series.assign(s->n_inputs, {});

@@ -66,6 +66,7 @@ pub struct NhConfig {
     pub step_shape: i32,
     pub graph_steps: u32,
     pub block: u32,
+    pub lds_bins: u32,
 }
 
 /// `nh_ne2_info`: the `Meta` and table sizes of a loaded `.ne2` file.

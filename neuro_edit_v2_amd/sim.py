@@ -40,6 +40,7 @@ class SimConfig:
     step_shape: int = -1
     graph_steps: int = 0
     block: int = 0
+    lds_bins: int = 0
 
     def to_c(self):
         c = L.Config()
