@@ -459,10 +459,12 @@ __device__ __forceinline__ void spike_words(const StepArgs &a, uint32_t wave, ui
 // 9 bands 24.2, 17 23.5, 33 23.3, 65 24.1).  Any grid size gives the same
 // results: the map is a permutation of the grid's workgroups.
 constexpr uint32_t kBands = 33;
+constexpr bool kLateRows = true;
 
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
 __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     using RT = typename Ring<RING>::T;
+    constexpr bool LATE_ROWS = LEAN && kLateRows;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t blk = blockIdx.x;
     if (!LEAN && PUSH && !a.single_net && gridDim.x % kBands == 0u)
@@ -491,8 +493,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     const float s_old = a.s[i];
     float2 xs = a.x[i];
     const float ext = a.ext ? a.ext[i] : 0.0f;
+    // CSR row bounds: loaded with the state, except in the lean variant (one
+    // large network), where only a wave that spikes loads them, after its
+    // ballot -- at 1M neurons about 3 % of the waves, 4 B per neuron-step
+    // of HBM traffic saved against one more load round trip for those waves
     uint32_t row_lo = 0, row_hi = 0;
-    if (PUSH) {
+    if (PUSH && !LATE_ROWS) {
         row_lo = a.row_lo[i];
         row_hi = a.row_hi[i];
     }
@@ -519,6 +525,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     // ---- wave64 ballot: the mask IS this wave's word of the spike bitmap
     const uint64_t mask = __ballot(spiked);
     if (PUSH && mask) {
+        if (LATE_ROWS) {
+            row_lo = a.row_lo[i];
+            row_hi = a.row_hi[i];
+        }
         // The push goes first, before any store of this step: a wave's
         // vector memory operations complete in order, so rows read after
         // the stores would wait for the stores.  The push writes slots
