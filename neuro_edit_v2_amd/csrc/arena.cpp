@@ -879,7 +879,7 @@ int Arena::check_pair_abort() {
 
 // Enqueue k steps starting at arena step t0 (t0 known to the host; the
 // kernels read the absolute step from d_t, which holds t0 at this point).
-int Arena::launch_chunk(uint64_t t0, uint32_t k) {
+int Arena::launch_chunk(uint64_t t0, uint32_t k, hipEvent_t start, hipEvent_t stop) {
     StepArgs a = args();
     TRY(check_args(a));
     const uint32_t nb = neuron_blocks(), sb = scat_blocks();
@@ -887,17 +887,26 @@ int Arena::launch_chunk(uint64_t t0, uint32_t k) {
     for (uint32_t j = 0; j < k;) {
         a.k = j;
         a.slot = (uint32_t)((t0 + j) % D);
-        if (pair && j + 1 < k) {
+        const bool two = pair && j + 1 < k;
+        const bool ends = j + (two ? 2u : 1u) == k;
+        if (two) {
+            set_launch_events(j == 0 ? start : nullptr, ends ? stop : nullptr);
             launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, lean(), pair_w8, stream);
             j += 2;
             continue;
         }
+        // the two-kernel shape's step ends with its scatter kernel
+        set_launch_events(j == 0 ? start : nullptr, ends && fused ? stop : nullptr);
         if (binned()) launch_step_binned(a, nb, ring_int, pmode, mmode, packed != 0, stream);
         else launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, lean_step(), stream);
-        if (!fused) launch_scatter(a, sb, ring_int, packed != 0, stream);
+        if (!fused) {
+            set_launch_events(nullptr, ends ? stop : nullptr);
+            launch_scatter(a, sb, ring_int, packed != 0, stream);
+        }
         if (n_watch) launch_watch(a, stream);
         ++j;
     }
+    set_launch_events(nullptr, nullptr);
     launch_advance(d_t.as<uint64_t>(), k, stream);
     HIPCHK(hipGetLastError());
     return NH_OK;
@@ -905,12 +914,13 @@ int Arena::launch_chunk(uint64_t t0, uint32_t k) {
 
 // One persistent launch of k steps from arena step t0 (run_usable() true),
 // then the chunk advance of the device step counter.
-int Arena::launch_persistent(uint64_t t0, uint32_t k) {
+int Arena::launch_persistent(uint64_t t0, uint32_t k, hipEvent_t start, hipEvent_t stop) {
     StepArgs a = args();
     a.k = 0;
     a.slot = (uint32_t)(t0 % D);
     a.run_steps = k;
     TRY(check_args(a));
+    set_launch_events(start, stop);
     launch_run(a, neuron_blocks(), ring_int, pmode, mmode, packed != 0, stream);
     launch_advance(d_t.as<uint64_t>(), k, stream);
     HIPCHK(hipGetLastError());
@@ -1099,7 +1109,7 @@ int Arena::run(uint64_t n_steps) {
     if (run_usable()) {
         // persistent kernel: one launch per chunk of steps, no graph
         if (n_steps == 0) return NH_OK;
-        HIPCHK(hipEventRecord(ev0, stream));
+        bool first = true;
         while (n_steps) {
             uint64_t k = std::min<uint64_t>(n_steps, run_max_steps);
             if (rast_steps) {
@@ -1110,18 +1120,21 @@ int Arena::run(uint64_t n_steps) {
                 }
                 k = std::min<uint64_t>(k, room);
             }
-            TRY(launch_persistent(t, (uint32_t)k));
+            TRY(launch_persistent(t, (uint32_t)k, first ? ev0 : nullptr, k == n_steps ? ev1 : nullptr));
+            first = false;
             pair_launched = true;
             t += k;
             n_steps -= k;
         }
-        HIPCHK(hipEventRecord(ev1, stream));
         timing_valid = true;
         return NH_OK;
     }
     if (use_graph && graph_dirty) TRY(capture_graph());
     if (n_steps == 0) return NH_OK;  // a zero-step run only prepares the graph
-    HIPCHK(hipEventRecord(ev0, stream));
+    // The run's GPU time (last_run_ms) spans its step kernels: the first
+    // one's start to the last one's end, taken by the kernels themselves on
+    // direct launches (a graph launch is bracketed by event records).
+    bool first = true;
     while (n_steps) {
         uint64_t k = std::min<uint64_t>(n_steps, stim_cap);
         bool graph_ok = use_graph != 0 && t % D == 0;
@@ -1136,13 +1149,19 @@ int Arena::run(uint64_t n_steps) {
             k = std::min<uint64_t>(k, room);
         }
         TRY(upload_stim(t, (uint32_t)k));
-        if (graph_ok && k == graph_steps) HIPCHK(hipGraphLaunch(graph_exec, stream));
-        else TRY(launch_chunk(t, (uint32_t)k));
+        const bool last = k == n_steps;
+        if (graph_ok && k == graph_steps) {
+            if (first) HIPCHK(hipEventRecord(ev0, stream));
+            HIPCHK(hipGraphLaunch(graph_exec, stream));
+            if (last) HIPCHK(hipEventRecord(ev1, stream));
+        } else {
+            TRY(launch_chunk(t, (uint32_t)k, first ? ev0 : nullptr, last ? ev1 : nullptr));
+        }
+        first = false;
         if (k > 1 && pair_usable()) pair_launched = true;
         t += k;
         n_steps -= k;
     }
-    HIPCHK(hipEventRecord(ev1, stream));
     timing_valid = true;
     return NH_OK;
 }

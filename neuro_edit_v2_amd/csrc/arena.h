@@ -30,6 +30,7 @@ uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool l
                              uint32_t block);
 void launch_run(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed, hipStream_t st);
 uint32_t run_resident_waves(int ring, int pmode, int mmode, bool packed, uint32_t block);
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
 void launch_count_rows(const uint64_t *bits, uint32_t R, uint32_t words, uint32_t base, uint32_t n,
                        uint64_t t, uint32_t rows, uint32_t *out, hipStream_t st);
 // Words of the file-scope count scratch (nh_spike_counts_noalloc), and its
@@ -254,8 +255,10 @@ class Arena {
     bool stage_used[2] = {false, false};
 
     StepArgs args() const;
-    int launch_chunk(uint64_t t0, uint32_t k);
-    int launch_persistent(uint64_t t0, uint32_t k);
+    // start: event recorded at the first step kernel's start; stop: at the
+    // last step kernel's end (either may be null)
+    int launch_chunk(uint64_t t0, uint32_t k, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+    int launch_persistent(uint64_t t0, uint32_t k, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
     int check_args(const StepArgs &a) const;
     // direct launches with an event pair around every step kernel
     int profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,

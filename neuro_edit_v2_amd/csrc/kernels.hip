@@ -29,6 +29,8 @@
 // every shape gives the CPU raster bit for bit whatever the atomic order.
 #include <hip/hip_runtime.h>
 
+#include <hip/hip_ext.h>
+
 #include <mutex>
 
 #include "device.h"
@@ -101,6 +103,26 @@ __device__ __forceinline__ uint32_t wave_rank(uint64_t mask) {
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Start/stop events for the next kernel launch (consumed by it): a run's
+// timing then spans its step kernels from the first one's start to the last
+// one's end (hipExtLaunchKernel records them at the kernel's own start and
+// end), without the host's enqueue latency before the first kernel.
+static thread_local hipEvent_t g_ev_start = nullptr, g_ev_stop = nullptr;
+
+void set_launch_events(hipEvent_t start, hipEvent_t stop) {
+    g_ev_start = start;
+    g_ev_stop = stop;
+}
+
+static void launch_fn(const void *fn, dim3 grid, dim3 block, void **args, size_t shmem, hipStream_t st) {
+    if (g_ev_start || g_ev_stop) {
+        (void)hipExtLaunchKernel(fn, grid, block, args, shmem, st, g_ev_start, g_ev_stop, 0);
+        g_ev_start = g_ev_stop = nullptr;
+    } else {
+        (void)hipLaunchKernel(fn, grid, block, args, shmem, st);
+    }
 }
 
 // Entries pushed per lane before the first atomic of a batch: every load of
@@ -1165,7 +1187,7 @@ static const void *run_kernel(int ring, int pmode, int mmode, bool packed) {
 
 void launch_run(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed, hipStream_t st) {
     void *args[] = {(void *)&a};
-    (void)hipLaunchKernel(run_kernel(ring, pmode, mmode, packed), dim3(blocks), dim3(a.block), args, 0, st);
+    launch_fn(run_kernel(ring, pmode, mmode, packed), dim3(blocks), dim3(a.block), args, 0, st);
 }
 
 // Waves of the persistent kernel the device holds at once with workgroups
@@ -1336,8 +1358,8 @@ static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
     // a packed arena's grid is padded to whole bands (waves past the arena
     // return at once)
     if (!LEAN && PUSH && !a.single_net && blocks >= 4u * kBands) blocks = (blocks + kBands - 1u) / kBands * kBands;
-    hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LEAN>), dim3(blocks), dim3(a.block), 0, st,
-                       a);
+    void *args[] = {(void *)&a};
+    launch_fn((const void *)k_step<RING, PMODE, MMODE, PUSH, PACKED, LEAN>, dim3(blocks), dim3(a.block), args, 0, st);
 }
 
 template <int RING, int PMODE, int MMODE>
@@ -1373,10 +1395,10 @@ template <int RING, int PMODE, int MMODE>
 static void go_binned(const StepArgs &a, uint32_t blocks, bool packed, hipStream_t st) {
     if (!a.single_net && blocks >= 4u * kBands) blocks = (blocks + kBands - 1u) / kBands * kBands;
     const uint32_t lds = binned_lds_bytes(a.bin_entries, a.block);
-    if (packed)
-        hipLaunchKernelGGL((k_step_binned<RING, PMODE, MMODE, true>), dim3(blocks), dim3(a.block), lds, st, a);
-    else
-        hipLaunchKernelGGL((k_step_binned<RING, PMODE, MMODE, false>), dim3(blocks), dim3(a.block), lds, st, a);
+    void *args[] = {(void *)&a};
+    launch_fn(packed ? (const void *)k_step_binned<RING, PMODE, MMODE, true>
+                     : (const void *)k_step_binned<RING, PMODE, MMODE, false>,
+              dim3(blocks), dim3(a.block), args, lds, st);
 }
 
 template <int RING>
@@ -1440,8 +1462,7 @@ static const void *pair_kernel(int ring, int pmode, int mmode, bool packed, bool
 void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
                  bool lean, bool w8, hipStream_t st) {
     void *args[] = {(void *)&a};
-    (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed, lean, w8 && lean), dim3(blocks),
-                          dim3(a.block), args, 0, st);
+    launch_fn(pair_kernel(ring, pmode, mmode, packed, lean, w8 && lean), dim3(blocks), dim3(a.block), args, 0, st);
 }
 
 // Waves of the paired-step kernel that the device holds at once with
@@ -1459,14 +1480,10 @@ uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool l
 }
 
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st) {
-    const dim3 b(a.block);
-    if (ring == 0) {
-        if (packed) hipLaunchKernelGGL((k_scatter<0, true>), dim3(blocks), b, 0, st, a);
-        else hipLaunchKernelGGL((k_scatter<0, false>), dim3(blocks), b, 0, st, a);
-    } else {
-        if (packed) hipLaunchKernelGGL((k_scatter<1, true>), dim3(blocks), b, 0, st, a);
-        else hipLaunchKernelGGL((k_scatter<1, false>), dim3(blocks), b, 0, st, a);
-    }
+    void *args[] = {(void *)&a};
+    const void *fn = ring == 0 ? (packed ? (const void *)k_scatter<0, true> : (const void *)k_scatter<0, false>)
+                               : (packed ? (const void *)k_scatter<1, true> : (const void *)k_scatter<1, false>);
+    launch_fn(fn, dim3(blocks), dim3(a.block), args, 0, st);
 }
 
 void launch_watch(const StepArgs &a, hipStream_t st) {
