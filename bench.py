@@ -309,6 +309,20 @@ def bench_large(args, kw):
                          "algorithmic_bytes_per_launch": int(kbytes)}}
 
 
+def settle(sim, steps, total):
+    """Clock-settle phase, untimed: the timed call run_steps(steps) repeated
+    until at least `total` steps have run.  Over the first ~6000 back-to-back
+    steps the device ramps to its steady clocks (100k: 2.85 -> 2.76 us per
+    step, flat after), so the driver's short warm-up alone would time the
+    ramp; and the library replays a chunk from a captured graph once it has
+    seen that chunk's length before, which the repeats give the timed call.
+    Returns the steps run."""
+    reps = max(2, -(-total // max(steps, 1)))
+    for _ in range(reps):
+        sim.run_steps(steps)
+    return reps * steps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,8 +359,9 @@ def main():
     ap.add_argument("--serve-check", type=int, default=10,
                     help="served requests re-run alone and compared with their served result")
     ap.add_argument("--settle", type=int, default=6000,
-                    help="untimed clock-settle steps after the warm-up (the device reaches its "
-                         "steady clocks over about 6000 back-to-back steps of this network)")
+                    help="untimed clock-settle steps after the warm-up, run as repeats of the "
+                         "timed call (the device reaches its steady clocks over about 6000 "
+                         "back-to-back steps of this network)")
     args = ap.parse_args()
     if args.copy_probe or args.launch_probe:
         import ctypes
@@ -388,12 +403,10 @@ def main():
     # the warm-up: reading them on the host here would leave the device idle
     # for a host round trip right before the timed steps (at 1M neurons a
     # 1.2 ms read made the next 400 steps 4-5 % slower: the clocks drop).
-    # Warm-up (the driver's W steps), then a clock-settle phase of its own:
-    # over the first ~6000 back-to-back steps the device ramps to its
-    # steady clocks (100k: 2.85 -> 2.76 us per step, flat after), so a short
-    # warm-up alone would time the ramp.
+    # Warm-up (the driver's W steps), then a clock-settle phase of its own
+    # (see settle()).
     sim.run_steps(args.warmup)
-    sim.run_steps(args.settle)
+    settled = settle(sim, args.steps, args.settle)
     sim.mark_counters()
     sim.synchronize()
     t0 = time.perf_counter()
@@ -480,7 +493,8 @@ def main():
     if not args.skip_other_ring:
         kw2 = dict(kw, int_weights=not kw["int_weights"])
         sim2 = Sim(net, SimConfig(exec_mode=args.exec_mode, step_shape=args.shape, **kw2))
-        sim2.run_steps(args.warmup + args.settle)
+        sim2.run_steps(args.warmup)
+        settle(sim2, args.steps, args.settle)
         sim2.mark_counters()
         sim2.synchronize()
         t1 = time.perf_counter()
@@ -539,7 +553,7 @@ def main():
         # step, beside the wall-clock figure above (which also holds the
         # host's launch and synchronisation of the timed region)
         "gpu_ms_per_step": round(gpu_ms / args.steps, 6),
-        "clock_settle_steps": args.settle,
+        "clock_settle_steps": settled,
         # the network arrives as host arrays: the rate with its upload counted
         "build_s": round(t_build, 4),
         "events_per_s_incl_build": round(events / (wall + t_build), 1),

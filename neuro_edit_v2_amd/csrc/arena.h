@@ -21,7 +21,7 @@ void launch_step_binned(const StepArgs &a, uint32_t blocks, int ring, int pmode,
                         hipStream_t st);
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st);
 void launch_watch(const StepArgs &a, hipStream_t st);
-void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
+void launch_advance(StepClock *c, uint32_t k, uint32_t D, hipStream_t st);
 void launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t lane_bytes, hipStream_t st);
 void launch_empty(uint32_t blocks, hipStream_t st);
 void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
@@ -30,7 +30,6 @@ uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool l
                              uint32_t block);
 void launch_run(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed, hipStream_t st);
 uint32_t run_resident_waves(int ring, int pmode, int mmode, bool packed, uint32_t block);
-void set_launch_events(hipEvent_t start, hipEvent_t stop);
 void launch_count_rows(const uint64_t *bits, uint32_t R, uint32_t words, uint32_t base, uint32_t n,
                        uint64_t t, uint32_t rows, uint32_t *out, hipStream_t st);
 // Words of the file-scope count scratch (nh_spike_counts_noalloc), and its
@@ -255,17 +254,27 @@ class Arena {
     bool stage_used[2] = {false, false};
 
     StepArgs args() const;
-    // start: event recorded at the first step kernel's start; stop: at the
-    // last step kernel's end (either may be null)
-    int launch_chunk(uint64_t t0, uint32_t k, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
-    int launch_persistent(uint64_t t0, uint32_t k, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+    int launch_chunk(uint64_t t0, uint32_t k, bool direct = true);
+    int launch_persistent(uint64_t t0, uint32_t k);
     int check_args(const StepArgs &a) const;
     // direct launches with an event pair around every step kernel
     int profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,
                 uint64_t &aux_n);
-    int capture_graph();
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
+    // Captured chunks, keyed by their length: a length is captured on its
+    // second use (or at once when forced), then replayed; at most kMaxGraphs
+    // are kept, the least recently used one is dropped first.
+    struct GraphChunk {
+        uint32_t k = 0;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        uint64_t used = 0;
+    };
+    static constexpr size_t kMaxGraphs = 16;
+    std::vector<GraphChunk> graphs;
+    std::vector<std::pair<uint32_t, uint32_t>> graph_seen;   // uncaptured lengths and their uses
+    uint64_t graph_clock = 0;
+    int graph_for(uint32_t k, bool force, hipGraphExec_t &out);
+    void drop_graphs();
 
     StimPacker packer_ = nullptr;
     void *packer_ctx_ = nullptr;

@@ -29,8 +29,6 @@
 // every shape gives the CPU raster bit for bit whatever the atomic order.
 #include <hip/hip_runtime.h>
 
-#include <hip/hip_ext.h>
-
 #include <mutex>
 
 #include "device.h"
@@ -105,23 +103,32 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-// Start/stop events for the next kernel launch (consumed by it): a run's
-// timing then spans its step kernels from the first one's start to the last
-// one's end (hipExtLaunchKernel records them at the kernel's own start and
-// end), without the host's enqueue latency before the first kernel.
-static thread_local hipEvent_t g_ev_start = nullptr, g_ev_stop = nullptr;
-
-void set_launch_events(hipEvent_t start, hipEvent_t stop) {
-    g_ev_start = start;
-    g_ev_stop = stop;
+// The absolute step at the start of the launch's chunk: by value for a
+// direct chunk, from the device clock for a captured one (a graph's nodes
+// are fixed, so the clock carries the step between replays).
+__device__ __forceinline__ uint64_t chunk_t(const StepArgs &a) {
+    return a.t_direct ? a.t_base : a.clock->t;
 }
 
-static void launch_fn(const void *fn, dim3 grid, dim3 block, void **args, size_t shmem, hipStream_t st) {
-    if (g_ev_start || g_ev_stop) {
-        (void)hipExtLaunchKernel(fn, grid, block, args, shmem, st, g_ev_start, g_ev_stop, 0);
-        g_ev_start = g_ev_stop = nullptr;
-    } else {
-        (void)hipLaunchKernel(fn, grid, block, args, shmem, st);
+// Ring slot of the launch's first step: fixed by the launcher for a direct
+// chunk; for a captured one the clock's slot plus the launch's offset
+// (k mod D), so that one graph per chunk length serves every start step.
+__device__ __forceinline__ uint32_t step_slot(const StepArgs &a) {
+    if (a.t_direct) return a.slot;
+    const uint32_t s = a.clock->slot + a.slot;
+    return s >= a.D ? s - a.D : s;
+}
+
+// The last launch of a direct chunk moves the clock on (no separate advance
+// kernel): no launch of a direct chunk reads the clock, so one thread may
+// store it at any point of the launch.  (A captured chunk ends with the
+// advance kernel: moving the clock from its last launch instead, by a
+// last-workgroup-out add, measured about 0.9 us slower per paired launch --
+// every launch of the kernel, not only the last.)
+__device__ __forceinline__ void advance_t(const StepArgs &a) {
+    if (a.t_advance && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.clock->t = a.t_next;
+        a.clock->slot = a.slot_next;
     }
 }
 
@@ -486,6 +493,7 @@ constexpr bool kLateRows = true;
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
 __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     using RT = typename Ring<RING>::T;
+    advance_t(a);
     constexpr bool LATE_ROWS = LEAN && kLateRows;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t blk = blockIdx.x;
@@ -494,7 +502,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     const uint32_t wave = blk * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
     if (wave >= a.words) return;
     const uint32_t i = wave * kWave + lane;
-    const uint32_t slot = a.slot;
+    const uint64_t t = chunk_t(a) + a.k;
+    const uint32_t slot = step_slot(a);
     // The network record is copied by value, so its fields are read once,
     // up front, into scalar registers (a reference that may point at either
     // the kernel arguments or the table makes every field a separate vector
@@ -526,7 +535,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     }
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
-    const uint64_t t = *a.t_dev + a.k;
     // raster window origin, read here (before any store of this kernel) so
     // it is a scalar load issued with the others, not a dependent vector
     // load after the state stores
@@ -620,6 +628,7 @@ __device__ __forceinline__ void ring_add_word(const StepArgs &a, uint32_t word, 
 template <int RING, int PMODE, int MMODE, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
     using RT = typename Ring<RING>::T;
+    advance_t(a);
     extern __shared__ uint32_t smem[];
     const uint32_t B = a.bin_entries;
     uint32_t *bkey = smem;
@@ -636,7 +645,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
     const uint32_t wave = blk * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
     const bool live = wave < a.words;              // every thread reaches the barriers
     const uint32_t i = wave * kWave + lane;
-    const uint32_t slot = a.slot;
+    const uint64_t t = chunk_t(a) + a.k;
+    const uint32_t slot = step_slot(a);
     uint64_t mask = 0;
     if (live) {
         NetRec nr;
@@ -656,7 +666,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
         const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
         uint32_t prow = 0;
         if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
-        const uint64_t t = *a.t_dev + a.k;
         const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
         const bool stim_on = a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
         const uint32_t local = i - nr.base;
@@ -807,12 +816,14 @@ __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t en
 template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
 __device__ __forceinline__ void pair_steps(const StepArgs &a) {
     using RT = typename Ring<RING>::T;
+    advance_t(a);
     __shared__ RT acc[kMaxBlock];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
     if (wave >= a.words) return;
     const uint32_t i = wave * kWave + lane;
-    const uint32_t slot0 = a.slot;
+    const uint64_t t0 = chunk_t(a) + a.k;
+    const uint32_t slot0 = step_slot(a);
     const uint32_t slot1 = slot0 + 1u == a.D ? 0u : slot0 + 1u;
     NetRec nr;
     uint32_t net = 0;
@@ -834,7 +845,6 @@ __device__ __forceinline__ void pair_steps(const StepArgs &a) {
     const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
-    const uint64_t t0 = *a.t_dev + a.k;
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
     const bool stim0 = !LEAN && a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
     const bool stim1 = !LEAN && a.stim_lo && a.stim_nz[(size_t)(a.k + 1) * a.stim_groups + net] != 0u;
@@ -1042,6 +1052,7 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, const uint64_t *pu
 template <int RING, int PMODE, int MMODE, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) void k_run(StepArgs a) {
     using RT = typename Ring<RING>::T;
+    advance_t(a);
     __shared__ RT acc[kMaxBlock];
     __shared__ uint32_t arrive[4];
     __shared__ uint32_t seen;
@@ -1068,7 +1079,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_run(StepArgs a) {
     const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
-    const uint64_t t0 = *a.t_dev + a.k;
+    const uint64_t t0 = chunk_t(a) + a.k;
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
     const uint32_t pl_lo = a.pl_ptr[wave], pl_hi = a.pl_hi[wave];
     const RT *pl_w = (const RT *)a.pl_w;
@@ -1085,7 +1096,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_run(StepArgs a) {
     RT *acc_w = acc + (threadIdx.x & ~(uint32_t)(kWave - 1));
     const uint32_t S = a.run_steps;
     const size_t pub_stride = 2u * (size_t)a.words;
-    uint32_t slot = a.slot;
+    uint32_t slot = step_slot(a);
 
     for (uint32_t k = 0; k < S; ++k) {
         const uint64_t t = t0 + k;
@@ -1187,7 +1198,7 @@ static const void *run_kernel(int ring, int pmode, int mmode, bool packed) {
 
 void launch_run(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed, hipStream_t st) {
     void *args[] = {(void *)&a};
-    launch_fn(run_kernel(ring, pmode, mmode, packed), dim3(blocks), dim3(a.block), args, 0, st);
+    (void)hipLaunchKernel(run_kernel(ring, pmode, mmode, packed), dim3(blocks), dim3(a.block), args, 0, st);
 }
 
 // Waves of the persistent kernel the device holds at once with workgroups
@@ -1206,26 +1217,29 @@ uint32_t run_resident_waves(int ring, int pmode, int mmode, bool packed, uint32_
 // Two-kernel shape: one wave per spike of step t, all synapses of the row.
 template <int RING, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) void k_scatter(StepArgs a) {
+    advance_t(a);
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
-    const uint64_t t = *a.t_dev + a.k;
+    const uint64_t t = chunk_t(a) + a.k;
+    const uint32_t slot = step_slot(a);
     const uint32_t L = (uint32_t)(t % 3);
     const uint32_t c = a.cnt[L];
     const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
     for (uint32_t q = wave; q < c; q += nwaves) {
         const uint32_t src = uniform(a.spk[(size_t)L * a.n_total + q]);
         const uint32_t lo = a.row_lo[src], hi = a.row_hi[src];
-        push_row<RING, PACKED>(a, lo, hi, a.slot, lane);
-        if (a.xt) push_custom<RING>(a, src, a.slot, lane);
+        push_row<RING, PACKED>(a, lo, hi, slot, lane);
+        if (a.xt) push_custom<RING>(a, src, slot, lane);
     }
 }
 
 // Watched-neuron state rows (sim/src/kernels.rs:141-147 moved out of the
 // per-neuron loop): one thread per watched neuron after the step.
 __global__ void k_watch(StepArgs a) {
+    advance_t(a);
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= a.n_watch) return;
-    const uint64_t t = *a.t_dev + a.k;
+    const uint64_t t = chunk_t(a) + a.k;
     a.xrec[(size_t)(t % a.xrec_depth) * a.n_watch + w] = a.x[a.watch[w]];
 }
 
@@ -1350,7 +1364,11 @@ void launch_empty(uint32_t blocks, hipStream_t st) {
 }
 
 // End of a chunk: advance the device step base.
-__global__ void k_advance(uint64_t *t_dev, uint32_t k) { *t_dev += k; }
+__global__ void k_advance(StepClock *c, uint32_t k, uint32_t kmod, uint32_t D) {
+    c->t += k;
+    const uint32_t s = c->slot + kmod;
+    c->slot = s >= D ? s - D : s;
+}
 
 // ------------------------------------------------------------ dispatch
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
@@ -1358,8 +1376,8 @@ static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
     // a packed arena's grid is padded to whole bands (waves past the arena
     // return at once)
     if (!LEAN && PUSH && !a.single_net && blocks >= 4u * kBands) blocks = (blocks + kBands - 1u) / kBands * kBands;
-    void *args[] = {(void *)&a};
-    launch_fn((const void *)k_step<RING, PMODE, MMODE, PUSH, PACKED, LEAN>, dim3(blocks), dim3(a.block), args, 0, st);
+    hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LEAN>), dim3(blocks), dim3(a.block), 0, st,
+                       a);
 }
 
 template <int RING, int PMODE, int MMODE>
@@ -1395,10 +1413,10 @@ template <int RING, int PMODE, int MMODE>
 static void go_binned(const StepArgs &a, uint32_t blocks, bool packed, hipStream_t st) {
     if (!a.single_net && blocks >= 4u * kBands) blocks = (blocks + kBands - 1u) / kBands * kBands;
     const uint32_t lds = binned_lds_bytes(a.bin_entries, a.block);
-    void *args[] = {(void *)&a};
-    launch_fn(packed ? (const void *)k_step_binned<RING, PMODE, MMODE, true>
-                     : (const void *)k_step_binned<RING, PMODE, MMODE, false>,
-              dim3(blocks), dim3(a.block), args, lds, st);
+    if (packed)
+        hipLaunchKernelGGL((k_step_binned<RING, PMODE, MMODE, true>), dim3(blocks), dim3(a.block), lds, st, a);
+    else
+        hipLaunchKernelGGL((k_step_binned<RING, PMODE, MMODE, false>), dim3(blocks), dim3(a.block), lds, st, a);
 }
 
 template <int RING>
@@ -1462,7 +1480,8 @@ static const void *pair_kernel(int ring, int pmode, int mmode, bool packed, bool
 void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
                  bool lean, bool w8, hipStream_t st) {
     void *args[] = {(void *)&a};
-    launch_fn(pair_kernel(ring, pmode, mmode, packed, lean, w8 && lean), dim3(blocks), dim3(a.block), args, 0, st);
+    (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed, lean, w8 && lean), dim3(blocks),
+                          dim3(a.block), args, 0, st);
 }
 
 // Waves of the paired-step kernel that the device holds at once with
@@ -1480,10 +1499,14 @@ uint32_t pair_resident_waves(int ring, int pmode, int mmode, bool packed, bool l
 }
 
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st) {
-    void *args[] = {(void *)&a};
-    const void *fn = ring == 0 ? (packed ? (const void *)k_scatter<0, true> : (const void *)k_scatter<0, false>)
-                               : (packed ? (const void *)k_scatter<1, true> : (const void *)k_scatter<1, false>);
-    launch_fn(fn, dim3(blocks), dim3(a.block), args, 0, st);
+    const dim3 b(a.block);
+    if (ring == 0) {
+        if (packed) hipLaunchKernelGGL((k_scatter<0, true>), dim3(blocks), b, 0, st, a);
+        else hipLaunchKernelGGL((k_scatter<0, false>), dim3(blocks), b, 0, st, a);
+    } else {
+        if (packed) hipLaunchKernelGGL((k_scatter<1, true>), dim3(blocks), b, 0, st, a);
+        else hipLaunchKernelGGL((k_scatter<1, false>), dim3(blocks), b, 0, st, a);
+    }
 }
 
 void launch_watch(const StepArgs &a, hipStream_t st) {
@@ -1491,8 +1514,8 @@ void launch_watch(const StepArgs &a, hipStream_t st) {
     hipLaunchKernelGGL(k_watch, dim3(blocks), dim3(256), 0, st, a);
 }
 
-void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st) {
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, t_dev, k);
+void launch_advance(StepClock *c, uint32_t k, uint32_t D, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, c, k, k % D, D);
 }
 
 }  // namespace nh
