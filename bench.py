@@ -16,6 +16,7 @@ one thread, on a bounded sample of the same workload).
                     [--ring int|float] [--no-cpu]
 """
 import argparse
+import gc
 import json
 import os
 import platform
@@ -285,11 +286,12 @@ def bench_large(args, kw):
     sim.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = sim.last_run_ms()
+    tail_ms, tail_n = sim.last_run_tail()
     spikes, events = sim.counts_since_mark()
     ev_step = events / steps
     spl = 2 if sim.launch_shape() == 2 else 1     # steps per launch
     kbytes = spl * (n * B_NEURON + ev_step * B_EVENT)
-    kavg_ms = gpu_ms / (steps / spl)
+    kavg_ms = tail_ms / tail_n if tail_n else gpu_ms / (steps / spl)
     achieved = kbytes / (kavg_ms * 1e-3) / 1e9
     traffic = load_traffic(f"traffic_fused_cube{n // 1000}k_{'int' if kw['int_weights'] else 'float'}.txt")
     return {"workload": f"random-cube Izhikevich {n} neurons x Poisson(100) out-synapses, "
@@ -316,11 +318,18 @@ def settle(sim, steps, total):
     step, flat after), so the driver's short warm-up alone would time the
     ramp; and the library replays a chunk from a captured graph once it has
     seen that chunk's length before, which the repeats give the timed call.
-    Returns the steps run."""
+    The last repeats each end in a synchronisation, as the timed call does,
+    so that the runtime's completion bookkeeping of the thousands of
+    launches before them is not left to the timed call (a single 20-step
+    call after the unsynchronised repeats measured 237 us of wall time
+    against 86 us typical).  Returns the steps run."""
     reps = max(2, -(-total // max(steps, 1)))
     for _ in range(reps):
         sim.run_steps(steps)
-    return reps * steps
+    for _ in range(5):
+        sim.run_steps(steps)
+        sim.synchronize()
+    return (reps + 5) * steps
 
 
 def main():
@@ -409,20 +418,26 @@ def main():
     settled = settle(sim, args.steps, args.settle)
     sim.mark_counters()
     sim.synchronize()
+    gc.disable()                         # no collector pause inside the timed region
     t0 = time.perf_counter()
     sim.run_steps(args.steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
+    gc.enable()
     gpu_ms = sim.last_run_ms()
+    tail_ms, tail_n = sim.last_run_tail()
     spikes, events = sim.counts_since_mark()
     value = events / wall
 
     # Dominant-kernel roofline.  Fused shape: the step kernel is the only
     # per-step launch, so its average launch duration is the timed region's
-    # HIP-event time (events on the sim's stream) over the launches (this
-    # includes the tiny chunk-advance kernel and any inter-launch gap, so it
-    # never flatters).  Two-kernel shape: a profiled pass with an event pair
-    # around every launch splits the step between the two kernels.
+    # HIP-event time (events on the sim's stream) over its launches, taken
+    # from the end of the region's first launch to the end of the region
+    # (last_run_tail): the launches back to back, without the host's latency
+    # before the first kernel; it still includes the tiny chunk-advance
+    # kernel and every inter-launch gap, so it never flatters.  Two-kernel
+    # shape: a profiled pass with an event pair around every launch splits
+    # the step between the two kernels.
     ev_per_step = events / args.steps
     shape = sim.launch_shape()           # 0 two kernels, 1 fused, 2 paired
     fused = shape != 0
@@ -444,7 +459,7 @@ def main():
                  3: "k_run (persistent: all steps of a run in one launch, delay-1 arrivals pulled)"}.get(
                      shape, "k_step (fused: update + own-spike push)")
         kbytes = spl * (args.n * B_NEURON + ev_per_step * B_EVENT)
-        kavg = gpu_ms / (args.steps / spl)
+        kavg = tail_ms / tail_n if tail_n else gpu_ms / (args.steps / spl)
     elif main_ms >= aux_ms:
         kname = "k_step (update + compaction)"
         kbytes = args.n * B_NEURON
@@ -460,6 +475,9 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
             "kernel": kname, "steps_per_launch": spl, "avg_launch_us": round(kavg * 1e3, 3),
+            "timed_launches": tail_n if (fused and tail_n) else None,
+            "avg_launch_window": ("timed region, end of its first launch to its end (launches back to "
+                                  "back)" if (fused and tail_n) else "timed region, whole"),
             "algorithmic_bytes_per_launch": int(kbytes),
             "bytes_per_neuron_step": B_NEURON, "bytes_per_event": B_EVENT,
             "events_per_step": round(ev_per_step, 1),
@@ -505,7 +523,8 @@ def main():
         sh2 = sim2.launch_shape()
         spl2 = 2 if sh2 == 2 else (min(args.steps, PERSIST_MAX) if sh2 == 3 else 1)
         kb2 = spl2 * (args.n * B_NEURON + ev2 / args.steps * B_EVENT)
-        ka2 = sim2.last_run_ms() / (args.steps / spl2)
+        t2_ms, t2_n = sim2.last_run_tail()
+        ka2 = t2_ms / t2_n if t2_n else sim2.last_run_ms() / (args.steps / spl2)
         other = {"ring": "int Q16.16" if kw2["int_weights"] else "float",
                  "value": round(ev2 / w2, 1), "unit": "synaptic events/s",
                  "ms_per_step": round(w2 / args.steps * 1e3, 5),

@@ -214,6 +214,14 @@ int nh_read_spikes(nh_sim *s, uint64_t *step0, uint32_t *counts, uint64_t max_st
 /* Device timing of the step kernels over the last nh_run_steps call
  * (hipEvents on the sim's stream): total milliseconds. */
 int nh_last_run_ms(nh_sim *s, double *ms);
+/* The last run's time after its first step launch (addition; no
+ * counterpart in the reference): milliseconds from the end of its first
+ * step launch (or first graph replay) to the end of the run, and the step
+ * launches in that span (a paired launch counts once).  This is the
+ * back-to-back time per launch without the host's latency before the first
+ * kernel.  *launches is 0 (and *ms the whole run's time) when the run had
+ * no launch after its first. */
+int nh_last_run_tail(nh_sim *s, double *ms, uint64_t *launches);
 /* Run n_steps with direct launches and a hipEvent pair around every
  * launch of the step kernels (on the sim's stream).  main_ms / main_n: the
  * neuron (or fused step) kernel; aux_ms / aux_n: the scatter kernel

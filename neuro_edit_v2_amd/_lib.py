@@ -113,6 +113,7 @@ def lib():
         "nh_grid_traces": (i32, [vp, f32, vp, u64]),
         "nh_read_spikes": (i32, [vp, P(u64), vp, u64, vp, u64, P(u64), P(u64)]),
         "nh_last_run_ms": (i32, [vp, P(f64)]),
+        "nh_last_run_tail": (i32, [vp, P(f64), P(u64)]),
         "nh_profile_steps": (i32, [vp, u64, P(f64), P(u64), P(f64), P(u64)]),
         "nh_autotune": (i32, [vp, u32, C.c_char_p, u32]),
         "nh_launch_shape": (i32, [vp, C.POINTER(C.c_int32)]),

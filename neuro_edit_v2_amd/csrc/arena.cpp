@@ -35,13 +35,15 @@ const char *last_error_text() { return g_err.c_str(); }
 
 Arena::~Arena() {
     if (stream) (void)hipStreamSynchronize(stream);
-    drop_graphs();
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     for (int i = 0; i < 2; ++i) {
         if (h_stage[i]) (void)hipHostFree(h_stage[i]);
         if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
     }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_tail) (void)hipEventDestroy(ev_tail);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -520,7 +522,7 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     if (any_stim) stim_ptr[n_total] = (uint32_t)stim_input.size();
 
     // ---- device buffers
-    TRY(alloc(d_t, sizeof(StepClock)));
+    TRY(alloc(d_t, 8));
     TRY(upload(d_nets, recs));
     TRY(upload(d_wave_net, wave_net));
     TRY(upload(d_x0, x0));
@@ -651,7 +653,7 @@ StepArgs Arena::args() const {
     a.n_total = n_total;
     a.block = block;
     a.D = D;
-    a.clock = d_t.as<StepClock>();
+    a.t_dev = d_t.as<uint64_t>();
     a.k = 0;
     a.slot = 0;
     a.single_net = (n_nets == 1 && !dynamic) ? 1 : 0;
@@ -747,7 +749,7 @@ int Arena::check_args(const StepArgs &a) const {
     auto need = [](const void *p, const char *what) -> int {
         return p ? NH_OK : set_error(NH_ERR_ARG, std::string("step argument missing: ") + what);
     };
-    TRY(need(a.clock, "clock"));
+    TRY(need(a.t_dev, "t_dev"));
     TRY(need(a.x, "x"));
     TRY(need(a.s, "s"));
     TRY(need(a.ring, "ring"));
@@ -876,67 +878,58 @@ int Arena::check_pair_abort() {
                                  "(workgroups not co-resident); the state is invalid, reset the simulation");
 }
 
-// Enqueue k steps starting at arena step t0.  A direct chunk gives the
-// kernels t0 by value and its last launch stores t0 + k into d_t; a
-// captured chunk (direct false: a graph replayed from any step) reads the
-// step from d_t, which holds the chunk's start step when it runs, takes the
-// ring slots from it, and ends with the advance kernel.
-int Arena::launch_chunk(uint64_t t0, uint32_t k, bool direct) {
+// Enqueue k steps starting at arena step t0 (t0 known to the host; the
+// kernels read the absolute step from d_t, which holds t0 at this point).
+int Arena::launch_chunk(uint64_t t0, uint32_t k) {
     StepArgs a = args();
     TRY(check_args(a));
     const uint32_t nb = neuron_blocks(), sb = scat_blocks();
     const bool pair = pair_usable();
-    if (direct) {
-        a.t_direct = 1;
-        a.t_base = t0;
-        a.t_next = t0 + k;
-        a.slot_next = (uint32_t)((t0 + k) % D);
-    }
     for (uint32_t j = 0; j < k;) {
         a.k = j;
         a.slot = (uint32_t)((t0 + j) % D);
-        const bool two = pair && j + 1 < k;
-        const bool ends = direct && j + (two ? 2u : 1u) == k;
-        if (two) {
-            a.t_advance = ends ? 1u : 0u;
+        if (pair && j + 1 < k) {
             launch_pair(a, nb, ring_int, pmode, mmode, packed != 0, lean(), pair_w8, stream);
+            TRY(note_launches(1));
             j += 2;
             continue;
         }
-        // the step's last launch: the watch copy, else the two-kernel
-        // shape's scatter, else the step kernel
-        a.t_advance = ends && fused && !n_watch ? 1u : 0u;
         if (binned()) launch_step_binned(a, nb, ring_int, pmode, mmode, packed != 0, stream);
         else launch_step(a, nb, ring_int, pmode, mmode, fused != 0, packed != 0, lean_step(), stream);
-        if (!fused) {
-            a.t_advance = ends && !n_watch ? 1u : 0u;
-            launch_scatter(a, sb, ring_int, packed != 0, stream);
-        }
-        if (n_watch) {
-            a.t_advance = ends ? 1u : 0u;
-            launch_watch(a, stream);
-        }
+        if (!fused) launch_scatter(a, sb, ring_int, packed != 0, stream);
+        if (n_watch) launch_watch(a, stream);
+        TRY(note_launches(1));
         ++j;
     }
-    if (!direct) launch_advance(d_t.as<StepClock>(), k, D, stream);
+    launch_advance(d_t.as<uint64_t>(), k, stream);
     HIPCHK(hipGetLastError());
     return NH_OK;
 }
 
-// One persistent launch of k steps from arena step t0 (run_usable() true);
-// the launch stores t0 + k into d_t.
+// After a run's step launches (not while a graph is captured): the first
+// one records ev_tail behind itself, later ones are counted.
+int Arena::note_launches(uint64_t n) {
+    if (!tail_marking) return NH_OK;
+    if (!tail_marked) {
+        HIPCHK(hipEventRecord(ev_tail, stream));
+        tail_marked = true;
+    } else {
+        tail_launches += n;
+    }
+    return NH_OK;
+}
+
+// One persistent launch of k steps from arena step t0 (run_usable() true),
+// then the chunk advance of the device step counter.
 int Arena::launch_persistent(uint64_t t0, uint32_t k) {
     StepArgs a = args();
     a.k = 0;
     a.slot = (uint32_t)(t0 % D);
     a.run_steps = k;
-    a.t_direct = 1;
-    a.t_base = t0;
-    a.t_next = t0 + k;
-    a.slot_next = (uint32_t)((t0 + k) % D);
-    a.t_advance = 1;
     TRY(check_args(a));
     launch_run(a, neuron_blocks(), ring_int, pmode, mmode, packed != 0, stream);
+    TRY(note_launches(1));
+    launch_advance(d_t.as<uint64_t>(), k, stream);
     HIPCHK(hipGetLastError());
     return NH_OK;
 }
@@ -1014,7 +1007,7 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
             if (n_watch) launch_watch(a, stream);
             ++j;
         }
-        launch_advance(d_t.as<StepClock>(), (uint32_t)k, D, stream);
+        launch_advance(d_t.as<uint64_t>(), (uint32_t)k, stream);
         hipError_t e = hipStreamSynchronize(stream);
         if (e != hipSuccess) {
             rc = set_error(NH_ERR_HIP, std::string("profile: ") + hipGetErrorString(e));
@@ -1039,69 +1032,28 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
     return rc;
 }
 
-void Arena::drop_graphs() {
-    for (GraphChunk &c : graphs) {
-        if (c.x) (void)hipGraphExecDestroy(c.x);
-        if (c.g) (void)hipGraphDestroy(c.g);
+// A captured chunk covers graph_steps steps, a multiple of the ring depth,
+// and is only replayed from arena steps t = 0 (mod D): the ring slot of
+// every step is then fixed per graph node.
+int Arena::capture_graph() {
+    tail_marking = false;   // nothing is recorded into the captured chunk
+    if (graph_exec) {
+        (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
     }
-    graphs.clear();
-    graph_seen.clear();
-}
-
-// The replayable graph of a k-step chunk (out stays null while the length
-// has been seen only once).  Its nodes are launch_chunk's launches in
-// captured form: they read the absolute step from d_t and take the ring
-// slot from it (t mod D), so the graph serves every start step; the chunk
-// ends with the advance kernel.
-int Arena::graph_for(uint32_t k, bool force, hipGraphExec_t &out) {
-    out = nullptr;
-    ++graph_clock;
-    for (GraphChunk &c : graphs)
-        if (c.k == k) {
-            c.used = graph_clock;
-            out = c.x;
-            return NH_OK;
-        }
-    const uint32_t key = k;
-    if (!force) {
-        auto it = std::find_if(graph_seen.begin(), graph_seen.end(),
-                               [&](const std::pair<uint32_t, uint32_t> &e) { return e.first == key; });
-        if (it == graph_seen.end()) {
-            if (graph_seen.size() >= 4 * kMaxGraphs) graph_seen.erase(graph_seen.begin());
-            graph_seen.push_back({key, 1u});
-            return NH_OK;
-        }
-        graph_seen.erase(it);
-    }
-    if (graphs.size() >= kMaxGraphs) {
-        auto lru = std::min_element(graphs.begin(), graphs.end(),
-                                    [](const GraphChunk &a, const GraphChunk &b) { return a.used < b.used; });
-        if (lru->x) (void)hipGraphExecDestroy(lru->x);
-        if (lru->g) (void)hipGraphDestroy(lru->g);
-        graphs.erase(lru);
+    if (graph) {
+        (void)hipGraphDestroy(graph);
+        graph = nullptr;
     }
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_chunk(0, k, false);
+    int rc = launch_chunk(0, graph_steps);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(stream, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-    }
+    if (rc) return rc;
     if (e != hipSuccess) return set_error(NH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    hipGraphExec_t x = nullptr;
-    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        return set_error(NH_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-    }
-    GraphChunk c;
-    c.k = k;
-    c.g = g;
-    c.x = x;
-    c.used = graph_clock;
-    graphs.push_back(c);
-    out = x;
+    graph = g;
+    HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    graph_dirty = false;
     return NH_OK;
 }
 
@@ -1151,11 +1103,11 @@ int Arena::run(uint64_t n_steps) {
         return NH_OK;
     }
     if (use_graph) {
-        // graph chunk: the requested length, even in the paired shape (so
-        // that every launch of a chunk is a pair)
+        // graph chunk: a multiple of D near the requested length (and even
+        // in the paired shape, so that every launch of a chunk is a pair)
         const uint32_t want = std::max<uint32_t>(graph_steps_req, 1);
-        const uint32_t unit = pair_usable() ? 2u : 1u;
-        const uint32_t g = (want + unit - 1) / unit * unit;
+        const uint32_t unit = (D % 2 && pair_usable()) ? 2 * D : D;
+        const uint32_t g = std::max<uint32_t>(1, (want + unit / 2) / unit) * unit;
         if (g != graph_steps) {
             graph_steps = g;
             graph_dirty = true;
@@ -1165,7 +1117,7 @@ int Arena::run(uint64_t n_steps) {
     if (run_usable()) {
         // persistent kernel: one launch per chunk of steps, no graph
         if (n_steps == 0) return NH_OK;
-        HIPCHK(hipEventRecord(ev0, stream));
+        TRY(begin_timing());
         while (n_steps) {
             uint64_t k = std::min<uint64_t>(n_steps, run_max_steps);
             if (rast_steps) {
@@ -1181,27 +1133,21 @@ int Arena::run(uint64_t n_steps) {
             t += k;
             n_steps -= k;
         }
-        HIPCHK(hipEventRecord(ev1, stream));
-        timing_valid = true;
-        return NH_OK;
+        return end_timing();
     }
-    if (graph_dirty) {
-        drop_graphs();
-        graph_dirty = false;
-    }
-    if (n_steps == 0) {
-        // a zero-step run prepares the graph of a whole chunk from here
-        if (use_graph) {
-            hipGraphExec_t x = nullptr;
-            TRY(graph_for(graph_steps, true, x));
-        }
-        return NH_OK;
-    }
-    HIPCHK(hipEventRecord(ev0, stream));
-    const uint64_t n_req = n_steps;
+    if (use_graph && graph_dirty) TRY(capture_graph());
+    if (n_steps == 0) return NH_OK;  // a zero-step run only prepares the graph
+    TRY(begin_timing());
     while (n_steps) {
         uint64_t k = std::min<uint64_t>(n_steps, stim_cap);
-        if (use_graph) k = std::min<uint64_t>(k, graph_steps);
+        // A captured chunk replays from steps t = 0 (mod D).  A run that
+        // reaches a whole chunk after the next such step realigns with a
+        // direct piece; a shorter one goes direct in one piece (so a short
+        // run in the paired shape is all pairs, whatever its start step).
+        const bool graph_ok = use_graph != 0 && t % D == 0;
+        if (use_graph && !graph_ok && n_steps >= (D - t % D) + graph_steps)
+            k = std::min<uint64_t>(k, D - t % D);
+        if (graph_ok) k = std::min<uint64_t>(k, graph_steps);
         if (rast_steps) {
             const uint64_t room = rast_steps - (t - rast_step0);
             if (room == 0) {
@@ -1211,18 +1157,33 @@ int Arena::run(uint64_t n_steps) {
             k = std::min<uint64_t>(k, room);
         }
         TRY(upload_stim(t, (uint32_t)k));
-        // Whole chunks, and a run short enough to be one chunk, replay the
-        // graph of their length once that length has been seen before
-        // (graph nodes run faster than the same kernels launched one by
-        // one); a long run's closing piece goes direct.
-        hipGraphExec_t x = nullptr;
-        if (use_graph && (k == graph_steps || k == n_req)) TRY(graph_for((uint32_t)k, false, x));
-        if (x) HIPCHK(hipGraphLaunch(x, stream));
-        else TRY(launch_chunk(t, (uint32_t)k));
+        if (graph_ok && k == graph_steps) {
+            HIPCHK(hipGraphLaunch(graph_exec, stream));
+            // a replay counts as its step launches (pairs: k / 2)
+            TRY(note_launches(pair_usable() ? k / 2 + k % 2 : k));
+        } else {
+            TRY(launch_chunk(t, (uint32_t)k));
+        }
         if (k > 1 && pair_usable()) pair_launched = true;
         t += k;
         n_steps -= k;
     }
+    return end_timing();
+}
+
+// Events of a run: ev0 before its first launch, ev_tail behind its first
+// step launch (note_launches), ev1 after its last.
+int Arena::begin_timing() {
+    if (!ev_tail) HIPCHK(hipEventCreate(&ev_tail));
+    HIPCHK(hipEventRecord(ev0, stream));
+    tail_marking = true;
+    tail_marked = false;
+    tail_launches = 0;
+    return NH_OK;
+}
+
+int Arena::end_timing() {
+    tail_marking = false;
     HIPCHK(hipEventRecord(ev1, stream));
     timing_valid = true;
     return NH_OK;
@@ -1233,6 +1194,12 @@ int Arena::sync() {
     if (timing_valid) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_run_ms = ms;
+        last_tail_ms = last_run_ms;
+        last_tail_launches = 0;
+        if (tail_marked && tail_launches && hipEventElapsedTime(&ms, ev_tail, ev1) == hipSuccess) {
+            last_tail_ms = ms;
+            last_tail_launches = tail_launches;
+        }
         timing_valid = false;
     }
     return check_pair_abort();
@@ -1257,7 +1224,7 @@ int Arena::reset() {
     HIPCHK(hipMemsetAsync(d_last.p, 0xFF, d_last.bytes, stream));
     HIPCHK(hipMemsetAsync(d_net_spikes.p, 0, d_net_spikes.bytes, stream));
     HIPCHK(hipMemsetAsync(d_rast_ctl.p, 0, d_rast_ctl.bytes, stream));
-    HIPCHK(hipMemsetAsync(d_t.p, 0, sizeof(StepClock), stream));
+    HIPCHK(hipMemsetAsync(d_t.p, 0, 8, stream));
     if (d_pub3.p) {
         HIPCHK(hipMemsetAsync(d_pub3.p, 0, d_pub3.bytes, stream));
         HIPCHK(hipMemsetAsync(d_gctr.p, 0, d_gctr.bytes, stream));

@@ -21,7 +21,7 @@ void launch_step_binned(const StepArgs &a, uint32_t blocks, int ring, int pmode,
                         hipStream_t st);
 void launch_scatter(const StepArgs &a, uint32_t blocks, int ring, bool packed, hipStream_t st);
 void launch_watch(const StepArgs &a, hipStream_t st);
-void launch_advance(StepClock *c, uint32_t k, uint32_t D, hipStream_t st);
+void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st);
 void launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t lane_bytes, hipStream_t st);
 void launch_empty(uint32_t blocks, hipStream_t st);
 void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
@@ -178,6 +178,19 @@ class Arena {
     int device = 0;            // the device current at creation
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_run_ms = 0.0;
+    // Run time after its first step launch: ev_tail is recorded right behind
+    // the run's first direct step launch (or its first graph replay), and
+    // the step launches after it are counted, so that ev_tail -> ev1 over
+    // that count is the back-to-back time per launch, without the host's
+    // latency before the first kernel.
+    hipEvent_t ev_tail = nullptr;
+    bool tail_marking = false, tail_marked = false;
+    uint64_t tail_launches = 0;
+    double last_tail_ms = 0.0;
+    uint64_t last_tail_launches = 0;
+    int note_launches(uint64_t n);
+    int begin_timing();
+    int end_timing();
     uint32_t stim_cols = 0, stim_cap = 0;
     uint32_t n_watch = 0, xrec_depth = 4096;
     bool has_ext = false;
@@ -254,27 +267,15 @@ class Arena {
     bool stage_used[2] = {false, false};
 
     StepArgs args() const;
-    int launch_chunk(uint64_t t0, uint32_t k, bool direct = true);
+    int launch_chunk(uint64_t t0, uint32_t k);
     int launch_persistent(uint64_t t0, uint32_t k);
     int check_args(const StepArgs &a) const;
     // direct launches with an event pair around every step kernel
     int profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,
                 uint64_t &aux_n);
-    // Captured chunks, keyed by their length: a length is captured on its
-    // second use (or at once when forced), then replayed; at most kMaxGraphs
-    // are kept, the least recently used one is dropped first.
-    struct GraphChunk {
-        uint32_t k = 0;
-        hipGraph_t g = nullptr;
-        hipGraphExec_t x = nullptr;
-        uint64_t used = 0;
-    };
-    static constexpr size_t kMaxGraphs = 16;
-    std::vector<GraphChunk> graphs;
-    std::vector<std::pair<uint32_t, uint32_t>> graph_seen;   // uncaptured lengths and their uses
-    uint64_t graph_clock = 0;
-    int graph_for(uint32_t k, bool force, hipGraphExec_t &out);
-    void drop_graphs();
+    int capture_graph();
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
 
     StimPacker packer_ = nullptr;
     void *packer_ctx_ = nullptr;

@@ -103,35 +103,6 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-// The absolute step at the start of the launch's chunk: by value for a
-// direct chunk, from the device clock for a captured one (a graph's nodes
-// are fixed, so the clock carries the step between replays).
-__device__ __forceinline__ uint64_t chunk_t(const StepArgs &a) {
-    return a.t_direct ? a.t_base : a.clock->t;
-}
-
-// Ring slot of the launch's first step: fixed by the launcher for a direct
-// chunk; for a captured one the clock's slot plus the launch's offset
-// (k mod D), so that one graph per chunk length serves every start step.
-__device__ __forceinline__ uint32_t step_slot(const StepArgs &a) {
-    if (a.t_direct) return a.slot;
-    const uint32_t s = a.clock->slot + a.slot;
-    return s >= a.D ? s - a.D : s;
-}
-
-// The last launch of a direct chunk moves the clock on (no separate advance
-// kernel): no launch of a direct chunk reads the clock, so one thread may
-// store it at any point of the launch.  (A captured chunk ends with the
-// advance kernel: moving the clock from its last launch instead, by a
-// last-workgroup-out add, measured about 0.9 us slower per paired launch --
-// every launch of the kernel, not only the last.)
-__device__ __forceinline__ void advance_t(const StepArgs &a) {
-    if (a.t_advance && blockIdx.x == 0 && threadIdx.x == 0) {
-        a.clock->t = a.t_next;
-        a.clock->slot = a.slot_next;
-    }
-}
-
 // Entries pushed per lane before the first atomic of a batch: every load of
 // a batch is in flight at once, so a batch costs one memory round trip.
 // A wave that gives up a bounded wait raises the abort flag: in device
@@ -493,7 +464,6 @@ constexpr bool kLateRows = true;
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
 __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     using RT = typename Ring<RING>::T;
-    advance_t(a);
     constexpr bool LATE_ROWS = LEAN && kLateRows;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t blk = blockIdx.x;
@@ -502,8 +472,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     const uint32_t wave = blk * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
     if (wave >= a.words) return;
     const uint32_t i = wave * kWave + lane;
-    const uint64_t t = chunk_t(a) + a.k;
-    const uint32_t slot = step_slot(a);
+    const uint32_t slot = a.slot;
     // The network record is copied by value, so its fields are read once,
     // up front, into scalar registers (a reference that may point at either
     // the kernel arguments or the table makes every field a separate vector
@@ -535,6 +504,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     }
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
+    const uint64_t t = *a.t_dev + a.k;
     // raster window origin, read here (before any store of this kernel) so
     // it is a scalar load issued with the others, not a dependent vector
     // load after the state stores
@@ -628,7 +598,6 @@ __device__ __forceinline__ void ring_add_word(const StepArgs &a, uint32_t word, 
 template <int RING, int PMODE, int MMODE, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
     using RT = typename Ring<RING>::T;
-    advance_t(a);
     extern __shared__ uint32_t smem[];
     const uint32_t B = a.bin_entries;
     uint32_t *bkey = smem;
@@ -645,8 +614,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
     const uint32_t wave = blk * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
     const bool live = wave < a.words;              // every thread reaches the barriers
     const uint32_t i = wave * kWave + lane;
-    const uint64_t t = chunk_t(a) + a.k;
-    const uint32_t slot = step_slot(a);
+    const uint32_t slot = a.slot;
     uint64_t mask = 0;
     if (live) {
         NetRec nr;
@@ -666,6 +634,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
         const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
         uint32_t prow = 0;
         if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
+        const uint64_t t = *a.t_dev + a.k;
         const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
         const bool stim_on = a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
         const uint32_t local = i - nr.base;
@@ -816,14 +785,12 @@ __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t en
 template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
 __device__ __forceinline__ void pair_steps(const StepArgs &a) {
     using RT = typename Ring<RING>::T;
-    advance_t(a);
     __shared__ RT acc[kMaxBlock];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
     if (wave >= a.words) return;
     const uint32_t i = wave * kWave + lane;
-    const uint64_t t0 = chunk_t(a) + a.k;
-    const uint32_t slot0 = step_slot(a);
+    const uint32_t slot0 = a.slot;
     const uint32_t slot1 = slot0 + 1u == a.D ? 0u : slot0 + 1u;
     NetRec nr;
     uint32_t net = 0;
@@ -845,6 +812,7 @@ __device__ __forceinline__ void pair_steps(const StepArgs &a) {
     const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
+    const uint64_t t0 = *a.t_dev + a.k;
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
     const bool stim0 = !LEAN && a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
     const bool stim1 = !LEAN && a.stim_lo && a.stim_nz[(size_t)(a.k + 1) * a.stim_groups + net] != 0u;
@@ -980,11 +948,11 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8))) 
 // invalidate for non-atomic data, of which there is none here.)
 // Every wait is bounded as in k_pair (pair_spins polls, then the abort
 // flag, which every other wait also checks): the launch always ends.
-// Measured on the 100k network: 3.9 us per step over long runs against
-// 2.77 for k_pair -- each step here chains about five memory round trips
-// across XCDs (gate, ring read, pull, push, completion) where a pair of
-// steps chains about four plus a launch -- so the launcher uses it for short
-// runs only, where one launch instead of many saves more than that.
+// Measured on the 100k network: 4.06 us per step over 2000-step runs and
+// 4.2 us per step over 20-step runs, against 2.87 and 3.34 for k_pair --
+// each step here chains about five memory round trips across XCDs (gate,
+// ring read, pull, push, completion) where a pair of steps chains about
+// four plus a launch -- so it runs only when asked for (step shape 3).
 
 // Ring word access at agent scope, as 32-bit words (float or int32 ring).
 template <typename RT>
@@ -1052,7 +1020,6 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, const uint64_t *pu
 template <int RING, int PMODE, int MMODE, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) void k_run(StepArgs a) {
     using RT = typename Ring<RING>::T;
-    advance_t(a);
     __shared__ RT acc[kMaxBlock];
     __shared__ uint32_t arrive[4];
     __shared__ uint32_t seen;
@@ -1079,7 +1046,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_run(StepArgs a) {
     const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
-    const uint64_t t0 = chunk_t(a) + a.k;
+    const uint64_t t0 = *a.t_dev + a.k;
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
     const uint32_t pl_lo = a.pl_ptr[wave], pl_hi = a.pl_hi[wave];
     const RT *pl_w = (const RT *)a.pl_w;
@@ -1096,7 +1063,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_run(StepArgs a) {
     RT *acc_w = acc + (threadIdx.x & ~(uint32_t)(kWave - 1));
     const uint32_t S = a.run_steps;
     const size_t pub_stride = 2u * (size_t)a.words;
-    uint32_t slot = step_slot(a);
+    uint32_t slot = a.slot;
 
     for (uint32_t k = 0; k < S; ++k) {
         const uint64_t t = t0 + k;
@@ -1217,29 +1184,26 @@ uint32_t run_resident_waves(int ring, int pmode, int mmode, bool packed, uint32_
 // Two-kernel shape: one wave per spike of step t, all synapses of the row.
 template <int RING, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) void k_scatter(StepArgs a) {
-    advance_t(a);
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
-    const uint64_t t = chunk_t(a) + a.k;
-    const uint32_t slot = step_slot(a);
+    const uint64_t t = *a.t_dev + a.k;
     const uint32_t L = (uint32_t)(t % 3);
     const uint32_t c = a.cnt[L];
     const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
     for (uint32_t q = wave; q < c; q += nwaves) {
         const uint32_t src = uniform(a.spk[(size_t)L * a.n_total + q]);
         const uint32_t lo = a.row_lo[src], hi = a.row_hi[src];
-        push_row<RING, PACKED>(a, lo, hi, slot, lane);
-        if (a.xt) push_custom<RING>(a, src, slot, lane);
+        push_row<RING, PACKED>(a, lo, hi, a.slot, lane);
+        if (a.xt) push_custom<RING>(a, src, a.slot, lane);
     }
 }
 
 // Watched-neuron state rows (sim/src/kernels.rs:141-147 moved out of the
 // per-neuron loop): one thread per watched neuron after the step.
 __global__ void k_watch(StepArgs a) {
-    advance_t(a);
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= a.n_watch) return;
-    const uint64_t t = chunk_t(a) + a.k;
+    const uint64_t t = *a.t_dev + a.k;
     a.xrec[(size_t)(t % a.xrec_depth) * a.n_watch + w] = a.x[a.watch[w]];
 }
 
@@ -1364,11 +1328,7 @@ void launch_empty(uint32_t blocks, hipStream_t st) {
 }
 
 // End of a chunk: advance the device step base.
-__global__ void k_advance(StepClock *c, uint32_t k, uint32_t kmod, uint32_t D) {
-    c->t += k;
-    const uint32_t s = c->slot + kmod;
-    c->slot = s >= D ? s - D : s;
-}
+__global__ void k_advance(uint64_t *t_dev, uint32_t k) { *t_dev += k; }
 
 // ------------------------------------------------------------ dispatch
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
@@ -1514,8 +1474,8 @@ void launch_watch(const StepArgs &a, hipStream_t st) {
     hipLaunchKernelGGL(k_watch, dim3(blocks), dim3(256), 0, st, a);
 }
 
-void launch_advance(StepClock *c, uint32_t k, uint32_t D, hipStream_t st) {
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, c, k, k % D, D);
+void launch_advance(uint64_t *t_dev, uint32_t k, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, t_dev, k);
 }
 
 }  // namespace nh

@@ -69,31 +69,14 @@ struct XArgs {
     const void *xs_w;
 };
 
-// The arena's step counter in device memory: the absolute step at which the
-// next chunk starts and its ring slot (step mod D), moved on after every
-// chunk.  Captured chunks read it; direct chunks carry both by value.
-struct alignas(16) StepClock {
-    uint64_t t;
-    uint32_t slot;
-    uint32_t pad;
-};
-
 // Kernel arguments of one step.  Pointers may be null when a feature is off.
 struct StepArgs {
     uint32_t n_total;            // arena neurons (multiple of 64)
     uint32_t block;              // workgroup size (64..512, multiple of 64)
     uint32_t D;                  // ring depth (max over packed networks)
-    StepClock *clock;            // step and slot at chunk start (device)
-    uint64_t t_base;             // t_direct: the chunk's start step, by value
-    uint64_t t_next;             // t_advance: the step after the chunk
-    uint32_t slot_next;          //    and its ring slot
-    uint32_t t_direct;           // 1: the step is t_base + k (clock is not read)
-    uint32_t t_advance;          // 1: this launch ends a direct chunk and stores
-                                 //    t_next, slot_next into the clock (no
-                                 //    advance kernel)
+    const uint64_t *t_dev;       // arena step of chunk start (device)
     uint32_t k;                  // step offset within the chunk
-    uint32_t slot;               // direct: (chunk start + k) % D; captured:
-                                 //    k % D, added to the clock's slot
+    uint32_t slot;               // (t_dev + k) % D, fixed by the launcher
     uint32_t single_net;         // 1: every wave uses net0 (no table read)
     const uint16_t *wave_net;    // per wave: index into nets
     const NetRec *nets;

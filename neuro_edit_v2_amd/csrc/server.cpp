@@ -288,7 +288,7 @@ int build_arena(nh_server *s) {
         ar.dev_bytes += b.bytes;
         return NH_OK;
     };
-    TRY(A(ar.d_t, sizeof(nh::StepClock)));
+    TRY(A(ar.d_t, 8));
     TRY(A(ar.d_nets, (uint64_t)c.max_requests * sizeof(nh::NetRec)));
     HIPCHK(hipMemcpy(ar.d_nets.p, ar.recs.data(), ar.d_nets.bytes, hipMemcpyHostToDevice));
     TRY(A(ar.d_wave_net, (uint64_t)ar.words * 2));

@@ -202,6 +202,14 @@ int nh_last_run_ms(nh_sim *s, double *ms) {
     return NH_OK;
 }
 
+int nh_last_run_tail(nh_sim *s, double *ms, uint64_t *launches) {
+    CHECK_SIM(s);
+    TRY(s->ar.sync());
+    *ms = s->ar.last_tail_ms;
+    *launches = s->ar.last_tail_launches;
+    return NH_OK;
+}
+
 int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *main_n, double *aux_ms,
                      uint64_t *aux_n) {
     CHECK_SIM(s);

@@ -212,6 +212,7 @@ extern "C" {
         n_ids: *mut u64,
     ) -> c_int;
     pub fn nh_last_run_ms(s: *mut NhSim, ms: *mut f64) -> c_int;
+    pub fn nh_last_run_tail(s: *mut NhSim, ms: *mut f64, launches: *mut u64) -> c_int;
     pub fn nh_profile_steps(
         s: *mut NhSim,
         n_steps: u64,

@@ -280,3 +280,10 @@ class Sim:
         v = C.c_double()
         L.check(L.lib().nh_last_run_ms(self._h, C.byref(v)))
         return v.value
+
+    def last_run_tail(self):
+        """(ms, launches): the last run's time after its first step launch
+        and the step launches in it (back-to-back time per launch)."""
+        v, n = C.c_double(), C.c_uint64()
+        L.check(L.lib().nh_last_run_tail(self._h, C.byref(v), C.byref(n)))
+        return v.value, int(n.value)
