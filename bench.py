@@ -434,8 +434,9 @@ def main():
     # HIP-event time (events on the sim's stream) over its launches, taken
     # from the end of the region's first launch to the end of the region
     # (last_run_tail): the launches back to back, without the host's latency
-    # before the first kernel; it still includes the tiny chunk-advance
-    # kernel and every inter-launch gap, so it never flatters.  Two-kernel
+    # before the first kernel; it includes every inter-launch gap and the
+    # chunk's closing step-counter advance, so it does not flatter the
+    # kernel.  Two-kernel
     # shape: a profiled pass with an event pair around every launch splits
     # the step between the two kernels.
     ev_per_step = events / args.steps
@@ -476,8 +477,8 @@ def main():
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
             "kernel": kname, "steps_per_launch": spl, "avg_launch_us": round(kavg * 1e3, 3),
             "timed_launches": tail_n if (fused and tail_n) else None,
-            "avg_launch_window": ("timed region, end of its first launch to its end (launches back to "
-                                  "back)" if (fused and tail_n) else "timed region, whole"),
+            "avg_launch_window": ("timed region, end of its first step launch to its end (launches back "
+                                  "to back)" if (fused and tail_n) else "timed region, whole"),
             "algorithmic_bytes_per_launch": int(kbytes),
             "bytes_per_neuron_step": B_NEURON, "bytes_per_event": B_EVENT,
             "events_per_step": round(ev_per_step, 1),

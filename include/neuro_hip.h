@@ -216,8 +216,9 @@ int nh_read_spikes(nh_sim *s, uint64_t *step0, uint32_t *counts, uint64_t max_st
 int nh_last_run_ms(nh_sim *s, double *ms);
 /* The last run's time after its first step launch (addition; no
  * counterpart in the reference): milliseconds from the end of its first
- * step launch (or first graph replay) to the end of the run, and the step
- * launches in that span (a paired launch counts once).  This is the
+ * step launch (or first graph replay) to the end of the run (which holds
+ * the chunks' one-thread step-counter advance launches as well), and the
+ * step launches in that span (a paired launch counts once).  This is the
  * back-to-back time per launch without the host's latency before the first
  * kernel.  *launches is 0 (and *ms the whole run's time) when the run had
  * no launch after its first. */
