@@ -461,18 +461,28 @@ __device__ __forceinline__ void spike_words(const StepArgs &a, uint32_t wave, ui
 constexpr uint32_t kBands = 33;
 constexpr bool kLateRows = true;
 
+// The first loads' pointers and indices come as leading scalar arguments,
+// preloaded into scalar registers (as for k_pair); band_len: workgroups per
+// band when the packed arena's grid is visited in interleaved bands, else 0.
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
-__global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
+__global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, float2 *x_p, uint32_t slot,
+                                                    uint32_t n_total, uint32_t words, uint32_t wpb,
+                                                    uint32_t band_len, StepArgs a) {
     using RT = typename Ring<RING>::T;
     constexpr bool LATE_ROWS = LEAN && kLateRows;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t blk = blockIdx.x;
-    if (!LEAN && PUSH && !a.single_net && gridDim.x % kBands == 0u)
-        blk = (blk % kBands) * (gridDim.x / kBands) + blk / kBands;
-    const uint32_t wave = blk * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
-    if (wave >= a.words) return;
+    if (!LEAN && PUSH && band_len) blk = (blk % kBands) * band_len + blk / kBands;
+    const uint32_t wave = blk * wpb + uniform(threadIdx.x >> 6);
+    if (wave >= words) return;
     const uint32_t i = wave * kWave + lane;
-    const uint32_t slot = a.slot;
+    // ---- first batch of independent loads, issued before the network
+    // record's (dependent) loads below
+    RT *ring = (RT *)ring_p;
+    const size_t ri = (size_t)slot * n_total + i;
+    const RT rv = ring[ri];
+    const float s_old = s_p[i];
+    float2 xs = x_p[i];
     // The network record is copied by value, so its fields are read once,
     // up front, into scalar registers (a reference that may point at either
     // the kernel arguments or the table makes every field a separate vector
@@ -485,13 +495,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
         net = uniform((uint32_t)a.wave_net[wave]);
         nr = a.nets[net];
     }
-
-    // ---- first batch of independent loads
-    RT *ring = (RT *)a.ring;
-    const size_t ri = (size_t)slot * a.n_total + i;
-    const RT rv = ring[ri];
-    const float s_old = a.s[i];
-    float2 xs = a.x[i];
     const float ext = a.ext ? a.ext[i] : 0.0f;
     // CSR row bounds: loaded with the state, except in the lean variant (one
     // large network), where only a wave that spikes loads them, after its
@@ -546,8 +549,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(StepArgs a) {
     // ---- this step's stores
     if (act) {
         ring[ri] = (RT)0;
-        a.x[i] = xs;
-        a.s[i] = s_new;
+        x_p[i] = xs;
+        s_p[i] = s_new;
         if (spiked) spike_marks<LEAN>(a, nr, i, t, tl);
     }
     if (lane == 0) spike_words(a, wave, t, rast_origin, mask);
@@ -782,16 +785,31 @@ __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t en
     if (has && (((uint32_t)v >> (src & 31u)) & 1u)) atomicAdd(&acc_w[ent >> 26], w);
 }
 
+// The values the first loads of a launch need come as separate leading
+// kernel arguments, which the build preloads into scalar registers
+// (-amdgpu-kernarg-preload-count): the state and ring loads issue at once,
+// without waiting for a load of the argument block (whose other fields are
+// needed only after those loads return).
 template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
-__device__ __forceinline__ void pair_steps(const StepArgs &a) {
+__device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p, uint32_t slot0,
+                                           uint32_t slot1, uint32_t n_total, uint32_t words,
+                                           uint32_t wpb, const uint64_t *t_dev, const StepArgs &a) {
     using RT = typename Ring<RING>::T;
     __shared__ RT acc[kMaxBlock];
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
-    if (wave >= a.words) return;
+    // waves per workgroup from the arguments (blockDim is a load of the
+    // dispatch's hidden arguments, which would delay the first loads)
+    const uint32_t wave = blockIdx.x * wpb + uniform(threadIdx.x >> 6);
+    if (wave >= words) return;
     const uint32_t i = wave * kWave + lane;
-    const uint32_t slot0 = a.slot;
-    const uint32_t slot1 = slot0 + 1u == a.D ? 0u : slot0 + 1u;
+    // ---- every load of the launch up front (the state first, then the
+    // network record's dependent loads)
+    RT *ring = (RT *)ring_p;
+    const size_t r0 = (size_t)slot0 * n_total + i, r1 = (size_t)slot1 * n_total + i;
+    const RT rv0 = ring[r0];
+    const RT rv1 = ring[r1];
+    float s = s_p[i];
+    float2 xs = x_p[i];
     NetRec nr;
     uint32_t net = 0;
     if (LEAN || a.single_net) {
@@ -800,19 +818,11 @@ __device__ __forceinline__ void pair_steps(const StepArgs &a) {
         net = uniform((uint32_t)a.wave_net[wave]);
         nr = a.nets[net];
     }
-
-    // ---- every load of the launch up front
-    RT *ring = (RT *)a.ring;
-    const size_t r0 = (size_t)slot0 * a.n_total + i, r1 = (size_t)slot1 * a.n_total + i;
-    const RT rv0 = ring[r0];
-    const RT rv1 = ring[r1];
-    float s = a.s[i];
-    float2 xs = a.x[i];
     const float ext = a.ext ? a.ext[i] : 0.0f;
     const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
-    const uint64_t t0 = *a.t_dev + a.k;
+    const uint64_t t0 = *t_dev + a.k;
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
     const bool stim0 = !LEAN && a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
     const bool stim1 = !LEAN && a.stim_lo && a.stim_nz[(size_t)(a.k + 1) * a.stim_groups + net] != 0u;
@@ -888,16 +898,18 @@ __device__ __forceinline__ void pair_steps(const StepArgs &a) {
     if (act0) ring[r0] = (RT)0;
     if (act1) ring[r1] = (RT)0;
     if (act0 || act1) {
-        a.x[i] = xs;
-        a.s[i] = s;
+        x_p[i] = xs;
+        s_p[i] = s;
     }
     if (sp1) spike_marks<LEAN>(a, nr, i, t1, tl1);
     if (lane == 0) spike_words(a, wave, t1, rast_origin, m1);
 }
 
 template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
-__global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
-    pair_steps<RING, PMODE, MMODE, PACKED, LEAN>(a);
+__global__ __launch_bounds__(kMaxBlock) void k_pair(void *ring, float *s, float2 *x, uint32_t slot0,
+                                                    uint32_t slot1, uint32_t n_total, uint32_t words,
+                                                    uint32_t wpb, const uint64_t *t_dev, StepArgs a) {
+    pair_steps<RING, PMODE, MMODE, PACKED, LEAN>(ring, s, x, slot0, slot1, n_total, words, wpb, t_dev, a);
 }
 
 // The same kernel held to the register budget of 8 waves per SIMD (the
@@ -907,8 +919,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(StepArgs a) {
 // launch; at 100k the spills it takes cost 0.14 us per step, so k_pair runs
 // there).
 template <int RING, int PMODE, int MMODE, bool PACKED>
-__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pair8(StepArgs a) {
-    pair_steps<RING, PMODE, MMODE, PACKED, true>(a);
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pair8(
+    void *ring, float *s, float2 *x, uint32_t slot0, uint32_t slot1, uint32_t n_total, uint32_t words,
+    uint32_t wpb, const uint64_t *t_dev, StepArgs a) {
+    pair_steps<RING, PMODE, MMODE, PACKED, true>(ring, s, x, slot0, slot1, n_total, words, wpb, t_dev, a);
 }
 
 // ---------------------------------------------------------------- persistent
@@ -1336,8 +1350,9 @@ static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
     // a packed arena's grid is padded to whole bands (waves past the arena
     // return at once)
     if (!LEAN && PUSH && !a.single_net && blocks >= 4u * kBands) blocks = (blocks + kBands - 1u) / kBands * kBands;
+    const uint32_t band_len = (!LEAN && PUSH && !a.single_net && blocks % kBands == 0u) ? blocks / kBands : 0u;
     hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LEAN>), dim3(blocks), dim3(a.block), 0, st,
-                       a);
+                       a.ring, a.s, a.x, a.slot, a.n_total, a.words, a.block / 64u, band_len, a);
 }
 
 template <int RING, int PMODE, int MMODE>
@@ -1439,7 +1454,13 @@ static const void *pair_kernel(int ring, int pmode, int mmode, bool packed, bool
 
 void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mmode, bool packed,
                  bool lean, bool w8, hipStream_t st) {
-    void *args[] = {(void *)&a};
+    void *ring_p = a.ring;
+    float *s_p = a.s;
+    float2 *x_p = a.x;
+    uint32_t slot0 = a.slot, slot1 = a.slot + 1u == a.D ? 0u : a.slot + 1u, n_total = a.n_total,
+             words = a.words, wpb = a.block / 64u;
+    const uint64_t *t_dev = a.t_dev;
+    void *args[] = {&ring_p, &s_p, &x_p, &slot0, &slot1, &n_total, &words, &wpb, &t_dev, (void *)&a};
     (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed, lean, w8 && lean), dim3(blocks),
                           dim3(a.block), args, 0, st);
 }
