@@ -852,6 +852,7 @@ bool Arena::run_usable() {
 bool Arena::pair_usable() {
     pair_w8 = false;
     if (fused != 2 || !pair_tables || n_watch || n_x || words == 0) return false;
+    if (D > 65536u) return false;   // the paired kernel takes its two ring slots as 16-bit halves
     if (pair_cap_block != block) {
         pair_cap_waves = pair_resident_waves(ring_int, pmode, mmode, packed != 0, lean(), false, block);
         pair_cap8_waves = lean() ? pair_resident_waves(ring_int, pmode, mmode, packed != 0, true, true, block) : 0;

@@ -461,16 +461,27 @@ __device__ __forceinline__ void spike_words(const StepArgs &a, uint32_t wave, ui
 constexpr uint32_t kBands = 33;
 constexpr bool kLateRows = true;
 
+// Leading-argument packing of the step kernels: the wave count and the
+// waves per workgroup in one word (wave count < 2^29, 1..8 waves), and the
+// parameter-row index array only where a kernel variant reads it.
+constexpr uint32_t kWpbShift = 29;
+constexpr uint32_t kWordsMask = (1u << kWpbShift) - 1u;
+static inline uint32_t pack_words_wpb(const StepArgs &a) { return a.words | (a.block / 64u - 1u) << kWpbShift; }
+static inline const uint8_t *prow_arg(const StepArgs &a, int pmode) {
+    return (pmode == 2 || (pmode == 0 && a.n_krows > 1)) ? a.prow : nullptr;
+}
+
 // The first loads' pointers and indices come as leading scalar arguments,
 // preloaded into scalar registers (as for k_pair); band_len: workgroups per
 // band when the packed arena's grid is visited in interleaved bands, else 0.
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
 __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, float2 *x_p, uint32_t slot,
-                                                    uint32_t n_total, uint32_t words, uint32_t wpb,
-                                                    uint32_t band_len, StepArgs a) {
+                                                    uint32_t n_total, uint32_t words_wpb, uint32_t band_len,
+                                                    const uint8_t *prow_p, StepArgs a) {
     using RT = typename Ring<RING>::T;
     constexpr bool LATE_ROWS = LEAN && kLateRows;
     const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t words = words_wpb & kWordsMask, wpb = (words_wpb >> kWpbShift) + 1u;
     uint32_t blk = blockIdx.x;
     if (!LEAN && PUSH && band_len) blk = (blk % kBands) * band_len + blk / kBands;
     const uint32_t wave = blk * wpb + uniform(threadIdx.x >> 6);
@@ -483,6 +494,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, fl
     const RT rv = ring[ri];
     const float s_old = s_p[i];
     float2 xs = x_p[i];
+    uint32_t prow = 0;
+    if (PMODE != 1 && prow_p) prow = prow_p[i];
     // The network record is copied by value, so its fields are read once,
     // up front, into scalar registers (a reference that may point at either
     // the kernel arguments or the table makes every field a separate vector
@@ -505,8 +518,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, fl
         row_lo = a.row_lo[i];
         row_hi = a.row_hi[i];
     }
-    uint32_t prow = 0;
-    if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
+    // (the step is read from the device counter even where the host knows
+    // it: given by value, it measured slower -- the noise hashes then get
+    // scheduled among the first loads)
     const uint64_t t = *a.t_dev + a.k;
     // raster window origin, read here (before any store of this kernel) so
     // it is a scalar load issued with the others, not a dependent vector
@@ -791,12 +805,14 @@ __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t en
 // without waiting for a load of the argument block (whose other fields are
 // needed only after those loads return).
 template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
-__device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p, uint32_t slot0,
-                                           uint32_t slot1, uint32_t n_total, uint32_t words,
-                                           uint32_t wpb, const uint64_t *t_dev, const StepArgs &a) {
+__device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p, uint32_t slots,
+                                           uint32_t n_total, uint32_t words_wpb, const uint8_t *prow_p,
+                                           const StepArgs &a) {
     using RT = typename Ring<RING>::T;
     __shared__ RT acc[kMaxBlock];
     const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t slot0 = slots & 0xFFFFu, slot1 = slots >> 16;
+    const uint32_t words = words_wpb & kWordsMask, wpb = (words_wpb >> kWpbShift) + 1u;
     // waves per workgroup from the arguments (blockDim is a load of the
     // dispatch's hidden arguments, which would delay the first loads)
     const uint32_t wave = blockIdx.x * wpb + uniform(threadIdx.x >> 6);
@@ -810,6 +826,8 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     const RT rv1 = ring[r1];
     float s = s_p[i];
     float2 xs = x_p[i];
+    uint32_t prow = 0;
+    if (PMODE != 1 && prow_p) prow = prow_p[i];
     NetRec nr;
     uint32_t net = 0;
     if (LEAN || a.single_net) {
@@ -820,9 +838,7 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     }
     const float ext = a.ext ? a.ext[i] : 0.0f;
     const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
-    uint32_t prow = 0;
-    if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
-    const uint64_t t0 = *t_dev + a.k;
+    const uint64_t t0 = *a.t_dev + a.k;
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
     const bool stim0 = !LEAN && a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
     const bool stim1 = !LEAN && a.stim_lo && a.stim_nz[(size_t)(a.k + 1) * a.stim_groups + net] != 0u;
@@ -906,10 +922,10 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
 }
 
 template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
-__global__ __launch_bounds__(kMaxBlock) void k_pair(void *ring, float *s, float2 *x, uint32_t slot0,
-                                                    uint32_t slot1, uint32_t n_total, uint32_t words,
-                                                    uint32_t wpb, const uint64_t *t_dev, StepArgs a) {
-    pair_steps<RING, PMODE, MMODE, PACKED, LEAN>(ring, s, x, slot0, slot1, n_total, words, wpb, t_dev, a);
+__global__ __launch_bounds__(kMaxBlock) void k_pair(void *ring, float *s, float2 *x, uint32_t slots,
+                                                    uint32_t n_total, uint32_t words_wpb, const uint8_t *prow,
+                                                    StepArgs a) {
+    pair_steps<RING, PMODE, MMODE, PACKED, LEAN>(ring, s, x, slots, n_total, words_wpb, prow, a);
 }
 
 // The same kernel held to the register budget of 8 waves per SIMD (the
@@ -920,9 +936,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(void *ring, float *s, float2
 // there).
 template <int RING, int PMODE, int MMODE, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pair8(
-    void *ring, float *s, float2 *x, uint32_t slot0, uint32_t slot1, uint32_t n_total, uint32_t words,
-    uint32_t wpb, const uint64_t *t_dev, StepArgs a) {
-    pair_steps<RING, PMODE, MMODE, PACKED, true>(ring, s, x, slot0, slot1, n_total, words, wpb, t_dev, a);
+    void *ring, float *s, float2 *x, uint32_t slots, uint32_t n_total, uint32_t words_wpb, const uint8_t *prow,
+    StepArgs a) {
+    pair_steps<RING, PMODE, MMODE, PACKED, true>(ring, s, x, slots, n_total, words_wpb, prow, a);
 }
 
 // ---------------------------------------------------------------- persistent
@@ -1352,7 +1368,7 @@ static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
     if (!LEAN && PUSH && !a.single_net && blocks >= 4u * kBands) blocks = (blocks + kBands - 1u) / kBands * kBands;
     const uint32_t band_len = (!LEAN && PUSH && !a.single_net && blocks % kBands == 0u) ? blocks / kBands : 0u;
     hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LEAN>), dim3(blocks), dim3(a.block), 0, st,
-                       a.ring, a.s, a.x, a.slot, a.n_total, a.words, a.block / 64u, band_len, a);
+                       a.ring, a.s, a.x, a.slot, a.n_total, pack_words_wpb(a), band_len, prow_arg(a, PMODE), a);
 }
 
 template <int RING, int PMODE, int MMODE>
@@ -1457,10 +1473,10 @@ void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mm
     void *ring_p = a.ring;
     float *s_p = a.s;
     float2 *x_p = a.x;
-    uint32_t slot0 = a.slot, slot1 = a.slot + 1u == a.D ? 0u : a.slot + 1u, n_total = a.n_total,
-             words = a.words, wpb = a.block / 64u;
-    const uint64_t *t_dev = a.t_dev;
-    void *args[] = {&ring_p, &s_p, &x_p, &slot0, &slot1, &n_total, &words, &wpb, &t_dev, (void *)&a};
+    uint32_t slots = a.slot | (a.slot + 1u == a.D ? 0u : a.slot + 1u) << 16, n_total = a.n_total,
+             words_wpb = pack_words_wpb(a);
+    const uint8_t *prow = prow_arg(a, pmode);
+    void *args[] = {&ring_p, &s_p, &x_p, &slots, &n_total, &words_wpb, &prow, (void *)&a};
     (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed, lean, w8 && lean), dim3(blocks),
                           dim3(a.block), args, 0, st);
 }
