@@ -654,6 +654,7 @@ StepArgs Arena::args() const {
     a.block = block;
     a.D = D;
     a.t_dev = d_t.as<uint64_t>();
+    a.self = d_args.as<StepArgs>();
     a.k = 0;
     a.slot = 0;
     a.single_net = (n_nets == 1 && !dynamic) ? 1 : 0;
@@ -750,6 +751,7 @@ int Arena::check_args(const StepArgs &a) const {
         return p ? NH_OK : set_error(NH_ERR_ARG, std::string("step argument missing: ") + what);
     };
     TRY(need(a.t_dev, "t_dev"));
+    TRY(need(a.self, "self"));
     TRY(need(a.x, "x"));
     TRY(need(a.s, "s"));
     TRY(need(a.ring, "ring"));
@@ -882,6 +884,7 @@ int Arena::check_pair_abort() {
 // Enqueue k steps starting at arena step t0 (t0 known to the host; the
 // kernels read the absolute step from d_t, which holds t0 at this point).
 int Arena::launch_chunk(uint64_t t0, uint32_t k) {
+    if (k > kMaxChunkSteps) return set_error(NH_ERR_ARG, "launch chunk longer than the step kernels take");
     StepArgs a = args();
     TRY(check_args(a));
     const uint32_t nb = neuron_blocks(), sb = scat_blocks();
@@ -943,6 +946,7 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
         t += n_steps;
         return NH_OK;
     }
+    TRY(sync_dev_args());
     constexpr uint32_t kPool = 64;  // steps per event batch
     std::vector<hipEvent_t> ev(4 * kPool, nullptr);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
@@ -1103,6 +1107,7 @@ int Arena::run(uint64_t n_steps) {
         last_run_ms = 0.0;
         return NH_OK;
     }
+    TRY(sync_dev_args());
     if (use_graph) {
         // graph chunk: a multiple of D near the requested length (and even
         // in the paired shape, so that every launch of a chunk is a pair)
@@ -1170,6 +1175,22 @@ int Arena::run(uint64_t n_steps) {
         n_steps -= k;
     }
     return end_timing();
+}
+
+int Arena::sync_dev_args() {
+    if (!d_args.p) {
+        TRY(alloc(d_args, sizeof(StepArgs)));
+        args_dev_valid = false;
+    }
+    StepArgs a = args();
+    a.k = 0;
+    a.slot = 0;
+    if (args_dev_valid && std::memcmp(&a, &args_dev_, sizeof(StepArgs)) == 0) return NH_OK;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(d_args.p, &a, sizeof(StepArgs), hipMemcpyHostToDevice));
+    args_dev_ = a;
+    args_dev_valid = true;
+    return NH_OK;
 }
 
 // Events of a run: ev0 before its first launch, ev_tail behind its first

@@ -267,6 +267,14 @@ class Arena {
     bool stage_used[2] = {false, false};
 
     StepArgs args() const;
+    // The device copy of args() that kernels taking a pointer read (k and
+    // slot zeroed): refreshed at the start of every run (and profile) when
+    // it differs, after the stream has drained (a kernel in flight may read
+    // the old copy).
+    DBuf d_args;
+    StepArgs args_dev_{};
+    bool args_dev_valid = false;
+    int sync_dev_args();
     int launch_chunk(uint64_t t0, uint32_t k);
     int launch_persistent(uint64_t t0, uint32_t k);
     int check_args(const StepArgs &a) const;

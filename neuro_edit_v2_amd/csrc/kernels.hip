@@ -466,6 +466,11 @@ constexpr bool kLateRows = true;
 // parameter-row index array only where a kernel variant reads it.
 constexpr uint32_t kWpbShift = 29;
 constexpr uint32_t kWordsMask = (1u << kWpbShift) - 1u;
+// k_step's slot / step-offset / waves-per-workgroup word: slot < 2^17 (the
+// ring holds at most 65,537 slots), step offset < 2^12 (kMaxChunkSteps)
+constexpr uint32_t kSlotBits = 17;
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
+constexpr uint32_t kKMask = (1u << (kWpbShift - kSlotBits)) - 1u;
 static inline uint32_t pack_words_wpb(const StepArgs &a) { return a.words | (a.block / 64u - 1u) << kWpbShift; }
 static inline const uint8_t *prow_arg(const StepArgs &a, int pmode) {
     return (pmode == 2 || (pmode == 0 && a.n_krows > 1)) ? a.prow : nullptr;
@@ -475,13 +480,15 @@ static inline const uint8_t *prow_arg(const StepArgs &a, int pmode) {
 // preloaded into scalar registers (as for k_pair); band_len: workgroups per
 // band when the packed arena's grid is visited in interleaved bands, else 0.
 template <int RING, int PMODE, int MMODE, bool PUSH, bool PACKED, bool LEAN>
-__global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, float2 *x_p, uint32_t slot,
-                                                    uint32_t n_total, uint32_t words_wpb, uint32_t band_len,
-                                                    const uint8_t *prow_p, StepArgs a) {
+__global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, float2 *x_p, uint32_t n_total,
+                                                    uint32_t lk, uint32_t band_len, const uint8_t *prow_p,
+                                                    const StepArgs *__restrict__ pa) {
     using RT = typename Ring<RING>::T;
+    const StepArgs &a = *pa;
     constexpr bool LATE_ROWS = LEAN && kLateRows;
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t words = words_wpb & kWordsMask, wpb = (words_wpb >> kWpbShift) + 1u;
+    const uint32_t words = n_total >> 6;
+    const uint32_t slot = lk & kSlotMask, k = (lk >> kSlotBits) & kKMask, wpb = (lk >> kWpbShift) + 1u;
     uint32_t blk = blockIdx.x;
     if (!LEAN && PUSH && band_len) blk = (blk % kBands) * band_len + blk / kBands;
     const uint32_t wave = blk * wpb + uniform(threadIdx.x >> 6);
@@ -521,14 +528,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, fl
     // (the step is read from the device counter even where the host knows
     // it: given by value, it measured slower -- the noise hashes then get
     // scheduled among the first loads)
-    const uint64_t t = *a.t_dev + a.k;
+    const uint64_t t = *a.t_dev + k;
     // raster window origin, read here (before any store of this kernel) so
     // it is a scalar load issued with the others, not a dependent vector
     // load after the state stores
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
     // does this network have a nonzero input this step?  (scalar load,
     // issued with the others)
-    const bool stim_on = !LEAN && a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
+    const bool stim_on = !LEAN && a.stim_lo && a.stim_nz[(size_t)k * a.stim_groups + net] != 0u;
 
     const uint32_t local = i - nr.base;
     const uint64_t tl = t - (uint64_t)nr.t0;
@@ -537,7 +544,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, fl
     float s_new = s_old;
     if (act)
         spiked = neuron_update<RING, PMODE, MMODE, false, LEAN, false>(
-            a, nr, i, local, tl, slot, a.k, stim_on, prow, rv, ext, Noise{false, 0.0f}, Params{}, s_new, xs);
+            a, nr, i, local, tl, slot, k, stim_on, prow, rv, ext, Noise{false, 0.0f}, Params{}, s_new, xs);
 
     // ---- wave64 ballot: the mask IS this wave's word of the spike bitmap
     const uint64_t mask = __ballot(spiked);
@@ -806,8 +813,8 @@ __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t en
 // needed only after those loads return).
 template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
 __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p, uint32_t slots,
-                                           uint32_t n_total, uint32_t words_wpb, const uint8_t *prow_p,
-                                           const StepArgs &a) {
+                                           uint32_t n_total, uint32_t words_wpb, uint32_t k,
+                                           const uint8_t *prow_p, const StepArgs &a) {
     using RT = typename Ring<RING>::T;
     __shared__ RT acc[kMaxBlock];
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -838,10 +845,10 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     }
     const float ext = a.ext ? a.ext[i] : 0.0f;
     const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
-    const uint64_t t0 = *a.t_dev + a.k;
+    const uint64_t t0 = *a.t_dev + k;
     const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
-    const bool stim0 = !LEAN && a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
-    const bool stim1 = !LEAN && a.stim_lo && a.stim_nz[(size_t)(a.k + 1) * a.stim_groups + net] != 0u;
+    const bool stim0 = !LEAN && a.stim_lo && a.stim_nz[(size_t)k * a.stim_groups + net] != 0u;
+    const bool stim1 = !LEAN && a.stim_lo && a.stim_nz[(size_t)(k + 1) * a.stim_groups + net] != 0u;
     // this wave's delay-1 in-table; the first 64 entries are loaded here
     const uint32_t pl_lo = a.pl_ptr[wave], pl_hi = a.pl_hi[wave];
     const RT *pl_w = (const RT *)a.pl_w;
@@ -861,7 +868,7 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     // the parameter row serves both steps
     const Params pp = select_params<PMODE, MMODE, true>(a, i, prow);
     if (act0)
-        sp0 = neuron_update<RING, PMODE, MMODE, false, LEAN, true>(a, nr, i, local, tl0, slot0, a.k, stim0,
+        sp0 = neuron_update<RING, PMODE, MMODE, false, LEAN, true>(a, nr, i, local, tl0, slot0, k, stim0,
                                                                   prow, rv0, ext, Noise{false, 0.0f}, pp,
                                                                   s, xs);
     const uint64_t m0 = __ballot(sp0);
@@ -905,7 +912,7 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     // ---- step t1
     bool sp1 = false;
     if (act1)
-        sp1 = neuron_update<RING, PMODE, MMODE, true, LEAN, true>(a, nr, i, local, tl1, slot1, a.k + 1u,
+        sp1 = neuron_update<RING, PMODE, MMODE, true, LEAN, true>(a, nr, i, local, tl1, slot1, k + 1u,
                                                                  stim1, prow, rv1p, ext, nz1, pp, s, xs);
     const uint64_t m1 = __ballot(sp1);
     if (m1) push_wave<RING, PACKED, false>(a, m1, row_lo, row_hi, slot1, lane);
@@ -923,9 +930,9 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
 
 template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
 __global__ __launch_bounds__(kMaxBlock) void k_pair(void *ring, float *s, float2 *x, uint32_t slots,
-                                                    uint32_t n_total, uint32_t words_wpb, const uint8_t *prow,
-                                                    StepArgs a) {
-    pair_steps<RING, PMODE, MMODE, PACKED, LEAN>(ring, s, x, slots, n_total, words_wpb, prow, a);
+                                                    uint32_t n_total, uint32_t words_wpb, uint32_t k,
+                                                    const uint8_t *prow, const StepArgs *__restrict__ pa) {
+    pair_steps<RING, PMODE, MMODE, PACKED, LEAN>(ring, s, x, slots, n_total, words_wpb, k, prow, *pa);
 }
 
 // The same kernel held to the register budget of 8 waves per SIMD (the
@@ -936,9 +943,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(void *ring, float *s, float2
 // there).
 template <int RING, int PMODE, int MMODE, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pair8(
-    void *ring, float *s, float2 *x, uint32_t slots, uint32_t n_total, uint32_t words_wpb, const uint8_t *prow,
-    StepArgs a) {
-    pair_steps<RING, PMODE, MMODE, PACKED, true>(ring, s, x, slots, n_total, words_wpb, prow, a);
+    void *ring, float *s, float2 *x, uint32_t slots, uint32_t n_total, uint32_t words_wpb, uint32_t k,
+    const uint8_t *prow, const StepArgs *__restrict__ pa) {
+    pair_steps<RING, PMODE, MMODE, PACKED, true>(ring, s, x, slots, n_total, words_wpb, k, prow, *pa);
 }
 
 // ---------------------------------------------------------------- persistent
@@ -1367,8 +1374,9 @@ static void go(const StepArgs &a, uint32_t blocks, hipStream_t st) {
     // return at once)
     if (!LEAN && PUSH && !a.single_net && blocks >= 4u * kBands) blocks = (blocks + kBands - 1u) / kBands * kBands;
     const uint32_t band_len = (!LEAN && PUSH && !a.single_net && blocks % kBands == 0u) ? blocks / kBands : 0u;
+    const uint32_t lk = a.slot | a.k << kSlotBits | (a.block / 64u - 1u) << kWpbShift;
     hipLaunchKernelGGL((k_step<RING, PMODE, MMODE, PUSH, PACKED, LEAN>), dim3(blocks), dim3(a.block), 0, st,
-                       a.ring, a.s, a.x, a.slot, a.n_total, pack_words_wpb(a), band_len, prow_arg(a, PMODE), a);
+                       a.ring, a.s, a.x, a.n_total, lk, band_len, prow_arg(a, PMODE), a.self);
 }
 
 template <int RING, int PMODE, int MMODE>
@@ -1476,7 +1484,9 @@ void launch_pair(const StepArgs &a, uint32_t blocks, int ring, int pmode, int mm
     uint32_t slots = a.slot | (a.slot + 1u == a.D ? 0u : a.slot + 1u) << 16, n_total = a.n_total,
              words_wpb = pack_words_wpb(a);
     const uint8_t *prow = prow_arg(a, pmode);
-    void *args[] = {&ring_p, &s_p, &x_p, &slots, &n_total, &words_wpb, &prow, (void *)&a};
+    uint32_t k = a.k;
+    const StepArgs *pa = a.self;
+    void *args[] = {&ring_p, &s_p, &x_p, &slots, &n_total, &words_wpb, &k, &prow, &pa};
     (void)hipLaunchKernel(pair_kernel(ring, pmode, mmode, packed, lean, w8 && lean), dim3(blocks),
                           dim3(a.block), args, 0, st);
 }

@@ -70,11 +70,18 @@ struct XArgs {
 };
 
 // Kernel arguments of one step.  Pointers may be null when a feature is off.
+// Steps of one launch chunk at most (the step kernels take a launch's step
+// offset within its chunk in 12 bits).
+constexpr uint32_t kMaxChunkSteps = 4096;
+
 struct StepArgs {
     uint32_t n_total;            // arena neurons (multiple of 64)
     uint32_t block;              // workgroup size (64..512, multiple of 64)
     uint32_t D;                  // ring depth (max over packed networks)
     const uint64_t *t_dev;       // arena step of chunk start (device)
+    const StepArgs *self;        // the arena's device copy of these arguments
+                                 // (all fields but k and slot, which the
+                                 // kernels that read the copy take apart)
     uint32_t k;                  // step offset within the chunk
     uint32_t slot;               // (t_dev + k) % D, fixed by the launcher
     uint32_t single_net;         // 1: every wave uses net0 (no table read)
