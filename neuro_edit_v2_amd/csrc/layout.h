@@ -55,18 +55,31 @@ constexpr int kKargRows = 4;        // parameter rows carried in the kernel argu
 // (k, delay, w) for the push.  Kept in device memory behind one pointer of
 // the step arguments: every wave reads its arguments with scalar loads, and
 // a group behind a pointer that may be null is read only where it is used.
+// Device-memory pointers of the kernel arguments.  In device code they carry
+// the global address space, so loads, stores and atomics through pointers
+// read out of memory (the argument block's device copy) compile to global
+// memory operations instead of flat ones (which count against the scalar
+// and LDS counters too, so every wait for a scalar load would also wait for
+// them).  Host code (and the host sources' device passes, which hold no
+// kernels) sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__) && defined(NH_DEVICE_KERNELS)
+template <class T> using gptr = __attribute__((address_space(1))) T *;
+#else
+template <class T> using gptr = T *;
+#endif
+
 struct XArgs {
     uint32_t n_x;
-    const uint32_t *xt_ptr;      // n_total+1
-    const uint32_t *xt_idx;
-    const float *xdecay;
-    float *xsyn;
-    void *xring;                 // D x n_x (float or int32)
-    const uint32_t *xs_lo;
-    const uint32_t *xs_hi;
-    const uint32_t *xs_k;
-    const uint16_t *xs_d;
-    const void *xs_w;
+    gptr<const uint32_t> xt_ptr;      // n_total+1
+    gptr<const uint32_t> xt_idx;
+    gptr<const float> xdecay;
+    gptr<float> xsyn;
+    gptr<void> xring;                 // D x n_x (float or int32)
+    gptr<const uint32_t> xs_lo;
+    gptr<const uint32_t> xs_hi;
+    gptr<const uint32_t> xs_k;
+    gptr<const uint16_t> xs_d;
+    gptr<const void> xs_w;
 };
 
 // Kernel arguments of one step.  Pointers may be null when a feature is off.
@@ -78,79 +91,79 @@ struct StepArgs {
     uint32_t n_total;            // arena neurons (multiple of 64)
     uint32_t block;              // workgroup size (64..512, multiple of 64)
     uint32_t D;                  // ring depth (max over packed networks)
-    const uint64_t *t_dev;       // arena step of chunk start (device)
+    gptr<const uint64_t> t_dev;       // arena step of chunk start (device)
     const StepArgs *self;        // the arena's device copy of these arguments
                                  // (all fields but k and slot, which the
                                  // kernels that read the copy take apart)
     uint32_t k;                  // step offset within the chunk
     uint32_t slot;               // (t_dev + k) % D, fixed by the launcher
     uint32_t single_net;         // 1: every wave uses net0 (no table read)
-    const uint16_t *wave_net;    // per wave: index into nets
-    const NetRec *nets;
+    gptr<const uint16_t> wave_net;    // per wave: index into nets
+    gptr<const NetRec> nets;
     NetRec net0;                 // the network of a single-network arena
     // state
-    float2 *x;                   // (v, u|w) per neuron
-    float *s;                    // decaying synaptic current
-    void *ring;                  // D x n_total (float or int32)
+    gptr<float2> x;                   // (v, u|w) per neuron
+    gptr<float> s;                    // decaying synaptic current
+    gptr<void> ring;                  // D x n_total (float or int32)
     uint32_t ring_bytes;         // ring size when it is below 2 GB (buffer atomics)
     uint32_t ring_buf;           // 1: deliveries use buffer atomics over ring_bytes
-    const float *ext;            // external current or null
+    gptr<const float> ext;            // external current or null
     // parameters
     uint32_t n_krows;            // rows in krow (1..kKargRows) in kernel-argument mode
     PRow krow[kKargRows];        // parameter rows carried in the arguments
-    const uint8_t *prow;         // per-neuron row index (table modes)
-    const PRow *ptab;            // global table (<= 256 rows)
-    const float *p;              // per-neuron rows (n_total x P) in row mode
-    const uint8_t *mreg;         // per-neuron registry model id (row mode)
+    gptr<const uint8_t> prow;         // per-neuron row index (table modes)
+    gptr<const PRow> ptab;            // global table (<= 256 rows)
+    gptr<const float> p;              // per-neuron rows (n_total x P) in row mode
+    gptr<const uint8_t> mreg;         // per-neuron registry model id (row mode)
     uint32_t P;
     // stimulus
-    const uint32_t *stim_lo;     // per-neuron stimulus entry range or null
-    const uint32_t *stim_hi;
-    const uint32_t *stim_input;
-    const float *stim_w;
-    const float *stim_win;       // [chunk step][columns]
-    const uint32_t *stim_nz;     // [chunk step][group]: 0 = every input of the group is zero
+    gptr<const uint32_t> stim_lo;     // per-neuron stimulus entry range or null
+    gptr<const uint32_t> stim_hi;
+    gptr<const uint32_t> stim_input;
+    gptr<const float> stim_w;
+    gptr<const float> stim_win;       // [chunk step][columns]
+    gptr<const uint32_t> stim_nz;     // [chunk step][group]: 0 = every input of the group is zero
     uint32_t stim_groups;
     uint32_t stim_cols;
     // spike outputs
-    uint32_t *spk;               // per-step spike lists (3 x n_total)
-    uint32_t *cnt;               // per-list counters
-    uint64_t *bits;              // record ring (rows x words) or null
+    gptr<uint32_t> spk;               // per-step spike lists (3 x n_total)
+    gptr<uint32_t> cnt;               // per-list counters
+    gptr<uint64_t> bits;              // record ring (rows x words) or null
     uint32_t bits_rows;
     uint32_t words;
-    uint32_t *pn_total;          // per-neuron lifetime spikes
-    uint32_t *last_spike;        // per-neuron last spike step (kNoSpike = never)
-    uint64_t *net_spikes;        // per-network spike totals or null
+    gptr<uint32_t> pn_total;          // per-neuron lifetime spikes
+    gptr<uint32_t> last_spike;        // per-neuron last spike step (kNoSpike = never)
+    gptr<uint64_t> net_spikes;        // per-network spike totals or null
     // raster stream
-    uint64_t *rast_bits;         // raster window: [step - origin][words] spike words, or null
-    uint64_t *rast_ctl;          // [1] step origin of the window, [2] overflow flag
+    gptr<uint64_t> rast_bits;         // raster window: [step - origin][words] spike words, or null
+    gptr<uint64_t> rast_ctl;          // [1] step origin of the window, [2] overflow flag
     uint32_t rast_cap_steps;     // rows of rast_bits
     // windowed spike counters [window][n_total] (served form) or null
-    uint32_t *wcnt;
+    gptr<uint32_t> wcnt;
     uint32_t n_win;
     // synapses (CSR by source, rows sorted by delay then target); the row
     // of neuron i is [row_lo[i], row_hi[i]) -- for a packed CSR row_hi is
     // row_lo + 1 (the same array), the served arena keeps both
-    const uint32_t *row_lo;
-    const uint32_t *row_hi;
-    const uint32_t *syn;         // packed (tgt | delay << 27) or tgt
-    const uint16_t *delay16;     // unpacked delays (null when packed)
-    const void *w;               // float or int32 per synapse
-    uint64_t *totals;            // spare counters (cleared with the arena)
-    const XArgs *xt;             // custom-tau synapses, or null (none)
+    gptr<const uint32_t> row_lo;
+    gptr<const uint32_t> row_hi;
+    gptr<const uint32_t> syn;         // packed (tgt | delay << 27) or tgt
+    gptr<const uint16_t> delay16;     // unpacked delays (null when packed)
+    gptr<const void> w;               // float or int32 per synapse
+    gptr<uint64_t> totals;            // spare counters (cleared with the arena)
+    gptr<const XArgs> xt;             // custom-tau synapses, or null (none)
     // paired-step kernel (two steps per launch): per wave, the delay-1
     // synapses into the wave ([pl_ptr[w], pl_hi[w]) of pl_ent = source |
     // target lane << 26, weight pl_w; pl_hi = pl_ptr + 1 for a packed
     // table, the served arena keeps both); the spike word each wave
     // publishes for the first step (pub[2w + half] = tag << 32 | half
     // word); the flag a wave raises when its bounded wait runs out
-    const uint32_t *pl_ptr;
-    const uint32_t *pl_hi;
-    const uint32_t *pl_ent;
-    const void *pl_w;
-    uint64_t *pub;
-    uint32_t *pair_abort;
-    uint32_t *pair_abort_host;   // the same flag in mapped host memory (read by the host without a copy)
+    gptr<const uint32_t> pl_ptr;
+    gptr<const uint32_t> pl_hi;
+    gptr<const uint32_t> pl_ent;
+    gptr<const void> pl_w;
+    gptr<uint64_t> pub;
+    gptr<uint32_t> pair_abort;
+    gptr<uint32_t> pair_abort_host;   // the same flag in mapped host memory (read by the host without a copy)
     uint32_t pair_spins;         // polls before a wave gives up waiting
     uint32_t pair_mute;          // test only: 1 = wave 0 never publishes
     // persistent kernel (many steps per launch): steps of the launch; the
@@ -160,17 +173,17 @@ struct StepArgs {
     // counter gctr[r][g] and its done tag done[r][g] = t + 1, published once
     // every workgroup of the group has finished step t (pushes complete)
     uint32_t run_steps;
-    uint64_t *pub3;
-    uint32_t *gctr;
-    uint32_t *done;
+    gptr<uint64_t> pub3;
+    gptr<uint32_t> gctr;
+    gptr<uint32_t> done;
     uint32_t n_groups;           // 1..64
     // LDS-binned push (k_step_binned): entries of the workgroup's bin table
     // (a power of two; 0 = the wave-by-wave push)
     uint32_t bin_entries;
     // watched neurons (record ring of state rows)
-    const uint32_t *watch;
+    gptr<const uint32_t> watch;
     uint32_t n_watch;
-    float2 *xrec;
+    gptr<float2> xrec;
     uint32_t xrec_depth;
 };
 
