@@ -45,7 +45,9 @@ typedef struct {
     const uint16_t *delay;     /* n_syn, steps >= 1 */
     const float *tau;          /* n_syn or NULL; > 0 = custom-tau synapse */
     uint32_t n_models;         /* model table size (<= 8) */
-    const uint8_t *model_ids;  /* registry id per table entry: 0 izh, 1 adex */
+    const uint8_t *model_ids;  /* registry id per table entry: 0 izh, 1 adex, 2 lif (the
+                                  reference's test-only plug-in, sim/tests/lif.rs:14-104:
+                                  row [tau, e, r, vth, vreset], state [v, unused]) */
     const uint8_t *model;      /* n, index into the model table */
     uint32_t n_params;         /* param row width (max over the table) */
     uint32_t n_state;          /* state row width (2 for izh and adex) */

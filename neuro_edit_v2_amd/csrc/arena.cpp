@@ -96,7 +96,7 @@ int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay) {
     if (d.n_params > (uint32_t)kMaxParams) return set_error(NH_ERR_ARG, "param row too wide");
     if (d.n > 0x7FFFFFC0u) return set_error(NH_ERR_CAPACITY, "network exceeds 2^31 neurons");
     for (uint32_t m = 0; m < d.n_models; ++m)
-        if (d.model_ids[m] > 1) return set_error(NH_ERR_ARG, "unknown model id " + std::to_string(d.model_ids[m]));
+        if (d.model_ids[m] > 2) return set_error(NH_ERR_ARG, "unknown model id " + std::to_string(d.model_ids[m]));
     // Per-neuron checks as reductions first (vectorised): the models in
     // use, whether any parameter or initial value has the all-ones exponent
     // of an infinity or NaN, whether src_ptr ever decreases; the loops that
@@ -109,7 +109,8 @@ int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay) {
     bool bad_model = d.n && mmax >= d.n_models;
     for (uint32_t m = 0; !bad_model && m < d.n_models; ++m)
         if ((used >> m) & 1u)
-            bad_model = (d.model_ids[m] == 1 && d.n_params < 10) || (d.model_ids[m] == 0 && d.n_params < 4);
+            bad_model = (d.model_ids[m] == 1 && d.n_params < 10) || (d.model_ids[m] == 0 && d.n_params < 4) ||
+                        (d.model_ids[m] == 2 && d.n_params < 5);
     for (uint32_t i = 0; bad_model && i < d.n; ++i) {
         if (d.model[i] >= d.n_models)
             return set_error(NH_ERR_ARG, "neuron " + std::to_string(i) + " names model " +
@@ -117,6 +118,7 @@ int check_desc(const nh_net_desc &d, const nh_config &c, uint32_t *max_delay) {
         const uint8_t id = d.model_ids[d.model[i]];
         if (id == 1 && d.n_params < 10) return set_error(NH_ERR_ARG, "adex rows need 10 params");
         if (id == 0 && d.n_params < 4) return set_error(NH_ERR_ARG, "izh rows need 4 params");
+        if (id == 2 && d.n_params < 5) return set_error(NH_ERR_ARG, "lif rows need 5 params");
     }
     auto all_finite = [](const float *v, uint64_t n) {
         uint32_t nonfin = 0;
@@ -210,7 +212,7 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     // ---- validate and lay out
     uint64_t off = 0;
     uint32_t maxd = 1;
-    bool any_izh = false, any_adex = false;
+    bool any_izh = false, any_adex = false, any_lif = false;
     uint32_t P = 4;
     uint64_t total_syn = 0;
     uint32_t cols = 0;
@@ -224,7 +226,8 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         for (uint32_t i = 0; i < d.n; ++i) {
             const uint8_t id = d.model_ids[d.model[i]];
             if (id == 0) any_izh = true;
-            else any_adex = true;
+            else if (id == 1) any_adex = true;
+            else any_lif = true;
         }
         const uint64_t E = d.n_syn;
         P = std::max(P, d.n_params);
@@ -245,7 +248,9 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
     // synapse indices are 32-bit on the device (and one value is reserved)
     if (total_syn >= 0xFFFF0000ull) return set_error(NH_ERR_CAPACITY, "arena exceeds 2^32 - 2^16 synapses");
     n_syn = total_syn;
-    mmode = any_adex ? (any_izh ? 2 : 1) : 0;
+    // 0: every neuron Izhikevich, 1: every neuron AdEx, 2: per-neuron
+    // registry id (mixed models, or any LIF)
+    mmode = any_lif ? 2 : (any_adex ? (any_izh ? 2 : 1) : 0);
     stim_cols = cols;
     packed = (n_total <= (1u << 27) && D <= 32) ? 1 : 0;
 

@@ -108,4 +108,15 @@ __device__ __forceinline__ bool adex_step(const float *p, float2 &x, float I, fl
     return sp;
 }
 
+// Leaky integrate-and-fire step (the reference's third model, a test-only
+// plug-in: sim/tests/lif.rs:14-104), row [tau, e, r, vth, vreset], state v in
+// x.x (x.y unused): reset for last step's crossing, integrate, spike test.
+__device__ __forceinline__ bool lif_step(const float *p, float2 &x, float I, float dt) {
+    float v = x.x;
+    if (v >= p[3]) v = p[4];
+    v += dt * (p[1] - v + p[2] * I) / p[0];
+    x.x = v;
+    return v >= p[3];
+}
+
 }  // namespace nh

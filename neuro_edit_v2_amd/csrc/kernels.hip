@@ -407,6 +407,8 @@ __device__ __forceinline__ bool neuron_update(const StepArgs &a, const NetRec &n
     }
     if (is_adex)
         spiked = adex_step(pr, xs, I, nr.dt);
+    else if (MMODE == 2 && model == 2)
+        spiked = lif_step(pr, xs, I, nr.dt);   // noise gain 1 (the trait's default)
     else
         spiked = izh_step(pr[0], pr[1], pr[2], pr[3], xs, I, nr.dt);
     s = sv * nr.decay_s;

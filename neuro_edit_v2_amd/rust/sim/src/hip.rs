@@ -63,13 +63,24 @@ impl HipSim {
         params.check(n).map_err(anyhow::Error::msg)?;
         let csr = net.synapse_csr(cfg.dt_ms).map_err(anyhow::Error::msg)?;
         let sm = StimMap::from_network(net);
-        // registry id per entry of the net's model table (model/mod.rs:118-125)
+        // registry id per entry of the net's model table (model/mod.rs:118-125);
+        // the LIF test model (sim/tests/lif.rs:14-104), which is not in the
+        // registry, is built into this backend as id 2
         let mut ids = Vec::with_capacity(params.models.len());
         for m in &params.models {
-            let (id, _) = crate::model::by_id(m.id())
-                .ok_or_else(|| anyhow::anyhow!("model {} is not in the registry", m.id()))?;
+            let id = match crate::model::by_id(m.id()) {
+                Some((id, _)) => id,
+                None if m.id() == "lif" => 2,
+                None => anyhow::bail!("model {} is not built into this backend", m.id()),
+            };
             ids.push(id);
         }
+        // state rows are two wide on the device (a LIF-only net has one slot)
+        let x0: std::borrow::Cow<[f32]> = if params.n_state >= 2 {
+            std::borrow::Cow::Borrowed(&params.x0)
+        } else {
+            std::borrow::Cow::Owned(params.x0.iter().flat_map(|&v| [v, 0.0]).collect())
+        };
         let any_tau = csr.tau_ms.iter().any(|&t| t > 0.0);
         let desc = ffi::NhNetDesc {
             n: n as u32,
@@ -83,9 +94,9 @@ impl HipSim {
             model_ids: ids.as_ptr(),
             model: params.model.as_ptr(),
             n_params: params.n_params as u32,
-            n_state: params.n_state as u32,
+            n_state: params.n_state.max(2) as u32,
             p: params.p.as_ptr(),
-            x0: params.x0.as_ptr(),
+            x0: x0.as_ptr(),
             n_inputs: net.inputs.len() as u32,
             stim_ptr: sm.neuron_ptr.as_ptr(),
             stim_input: sm.input_idx.as_ptr(),
@@ -271,7 +282,8 @@ impl Backend for HipSim {
         // [state slot][watch][samples], at most 4096 samples kept per watch
         const CAP: usize = 4096;
         let (nx, nw) = (self.n_state, self.n_watch);
-        let mut out = vec![0.0f32; nx * nw * CAP];
+        // the device keeps two state slots per watch (a LIF-only net reads one)
+        let mut out = vec![0.0f32; nx.max(2) * nw * CAP];
         let mut k = 0u32;
         check(unsafe { ffi::nh_drain_watch(self.h, out.as_mut_ptr(), CAP as u32, &mut k) })?;
         let k = k as usize;

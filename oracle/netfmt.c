@@ -151,7 +151,7 @@ void or_netb_set_model(or_netb *b, uint32_t id, int model, const float *params)
 {
     nb_neuron *r = &b->nr[id];
     r->model = model;
-    r->np = model == 1 ? 10 : 4;
+    r->np = model == 1 ? 10 : (model == 2 ? 5 : 4);
     memset(r->p, 0, sizeof(r->p));
     memcpy(r->p, params, sizeof(float) * r->np);
 }
@@ -394,7 +394,7 @@ void or_netb_params(const or_netb *b, uint32_t *n_models, uint8_t *model_ids, ui
         if (k == nm)
             model_ids[nm++] = (uint8_t)r->model;
         model[i] = (uint8_t)k;
-        uint32_t w = r->model == 1 ? 10 : 4;
+        uint32_t w = r->model == 1 ? 10 : (r->model == 2 ? 5 : 4);
         if (w > np)
             np = w;
         nx = 2;
@@ -406,6 +406,10 @@ void or_netb_params(const or_netb *b, uint32_t *n_models, uint8_t *model_ids, ui
         float *x = x0 + (uint64_t)i * nx;
         if (r->model == 1) {
             x[0] = r->p[2];
+            x[1] = 0.0f;
+        } else if (r->model == 2) {
+            /* the LIF test model's init (sim/tests/lif.rs:95-97): v = e */
+            x[0] = r->p[1];
             x[1] = 0.0f;
         } else {
             x[0] = -65.0f;

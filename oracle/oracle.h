@@ -14,6 +14,7 @@
  *   sim/src/state.rs:283-301     noise_hash / noise_fires
  *   sim/src/model/izh.rs:300-317 Izhikevich init + step
  *   sim/src/model/adex.rs:149-191 AdEx init + step + noise gain
+ *   sim/tests/lif.rs:14-104      the LIF test model (init, step; gain 1)
  *   sim/src/kernels.rs:57-66     int ring put/get (fixed point, lrintf)
  *   sim/src/stim.rs:34-127       StimMap / Stimulus::value
  *   net-format/src/rng.rs:17-81  xoshiro256** Rng
@@ -53,7 +54,7 @@ typedef struct {
     const uint16_t *delay;     /* n_syn, steps >= 1 */
     const float *tau;          /* n_syn or NULL (all 0 = merged path) */
     uint32_t n_models;         /* entries in model_ids (<= 8) */
-    const uint8_t *model_ids;  /* registry id per table entry: 0 izh, 1 adex */
+    const uint8_t *model_ids;  /* registry id per table entry: 0 izh, 1 adex, 2 lif */
     const uint8_t *model;      /* n: index into the table */
     uint32_t n_params, n_state;
     const float *p;            /* n * n_params */
@@ -141,7 +142,7 @@ void or_netb_set_speed(or_netb *b, float speed);
 /* kind: 0 excitatory, 1 inhibitory, 2 regular */
 uint32_t or_netb_add_neuron(or_netb *b, float x, float y, float z, int kind);
 void or_netb_set_kind(or_netb *b, uint32_t id, int kind);
-/* model: 0 izh (4 params), 1 adex (10 params) */
+/* model: 0 izh (4 params), 1 adex (10 params), 2 lif (5 params) */
 void or_netb_set_model(or_netb *b, uint32_t id, int model, const float *params);
 uint64_t or_netb_connect(or_netb *b, uint32_t src, uint32_t tgt, float w);
 void or_netb_push_synapse(or_netb *b, uint32_t src, uint32_t tgt, float w, float delay_ms,

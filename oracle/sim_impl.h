@@ -340,6 +340,13 @@ static uint32_t FN(sim_step)(SIMT *S)
             x[0] += dt * (LIT(0.04) * x[0] * x[0] + LIT(5.0) * x[0] + LIT(140.0) - x[1] + input);
             x[1] += dt * p[0] * (p[1] * x[0] - x[1]);
             sp = x[0] >= LIT(30.0);
+        } else if (mid == 2) {
+            /* the LIF test model (sim/tests/lif.rs:98-104): reset for last
+             * step's crossing, v += dt*(e - v + r*I)/tau, spike test */
+            if (x[0] >= p[3])
+                x[0] = p[4];
+            x[0] += dt * (p[1] - x[0] + p[2] * input) / p[0];
+            sp = x[0] >= p[3];
         } else {
             /* model/adex.rs:163-176 */
             if (x[0] >= p[9]) {

@@ -692,3 +692,59 @@ def test_set_synapse_tau_dist_draws_in_insertion_order():
     # insertion order of create_connections is source-major, so it equals CSR order
     assert np.array_equal(np.sort(got), np.sort(want))
     assert np.array_equal(got, want)
+
+
+# ---------------------------------------------------------------- LIF
+LIF_DEF = [20.0, -65.0, 1.0, -50.0, -70.0]     # [tau, e, r, vth, vreset], sim/tests/lif.rs:67
+
+
+def test_lif_hand_computed_trace():
+    """sim/tests/lif.rs:131-195 (the reference's test-only third model):
+    dt 1, v starts at e = -65 and stays there undriven; under drive 10,
+    v = -64.5 then -64.025; under drive 100 it crosses, the crossing step
+    leaves v above threshold (no reset yet), and the next step resets to
+    vreset before integrating: v = -70 + (-65 + 70 + 100)/20 = -64.75.
+    Without the entry reset v would rise from above -50; with the reset after
+    the step instead, the crossing step would already read -70."""
+    b = netb_line(1)
+    b.set_model(0, 2, LIF_DEF)
+    sim = O.OracleSim(b.build(1.0), O.default_cfg(dt_ms=1.0, record_spikes=1))
+    sim.run_steps(3)
+    assert float(sim.v()[0]) == -65.0
+    sim.set_external(0, 10.0)
+    sim.run_steps(1)
+    assert abs(np.float32(sim.v()[0]) - np.float32(-64.5)) < 1e-6   # f32, as lif.rs compares
+    sim.run_steps(1)
+    assert abs(np.float32(sim.v()[0]) - np.float32(-64.025)) < 1e-6
+    sim.set_external(0, 100.0)
+    for _ in range(40):
+        sim.run_steps(1)
+        if sim.spike_row(0)[0] & 1:
+            break
+    else:
+        raise AssertionError("LIF must spike under drive 100")
+    assert float(sim.v()[0]) >= -50.0
+    sim.run_steps(1)
+    assert abs(np.float32(sim.v()[0]) - np.float32(-64.75)) < 1e-6
+
+
+def test_lif_closed_form_step_and_noise_gain():
+    """One LIF step from v = e with input I is exactly
+    v = e + dt*(r*I)/tau (f32 arithmetic in the reference's order), and a
+    kick enters with gain 1 (the trait's default noise gain,
+    sim/src/model/mod.rs:103-106): with every step kicked, v moves by
+    kick/dt * r * dt / tau."""
+    b = netb_line(2)
+    b.set_model(0, 2, [10.0, -60.0, 2.0, -40.0, -75.0])
+    b.set_model(1, 2, [10.0, -60.0, 2.0, -40.0, -75.0])
+    sim = O.OracleSim(b.build(0.5), O.default_cfg(dt_ms=0.5))
+    sim.set_external(0, 7.0)
+    sim.run_steps(1)
+    e, dt, r, tau = np.float32(-60.0), np.float32(0.5), np.float32(2.0), np.float32(10.0)
+    want = e + dt * (e - e + r * np.float32(7.0)) / tau
+    assert float(sim.v()[0]) == float(want)
+    assert float(sim.v()[1]) == -60.0
+    kicked = O.OracleSim(b.build(0.5), O.default_cfg(dt_ms=0.5, noise_rate_hz=1e6, noise_kick_mv=3.0))
+    kicked.run_steps(1)
+    kick = np.float32(3.0) / dt
+    assert float(kicked.v()[0]) == float(e + dt * (e - e + r * kick) / tau)
