@@ -242,13 +242,18 @@ int nh_autotune(nh_sim *s, uint32_t trial_steps, char *report, uint32_t report_c
  * step, 1 one fused kernel per step, 2 two steps per launch, 3 many steps
  * per launch. */
 int nh_launch_shape(nh_sim *s, int32_t *shape);
-/* Test hook for the failure path of the two-steps-per-launch shape (no
- * counterpart in the reference): spin_limit = polls before a waiting wave
- * gives up (0 = the default), mute_wave0 = 1 makes wave 0 withhold its
- * published spike word, so the waves that wait for it give up, the launch
- * ends early and the next synchronising call reports NH_ERR_HIP.  Never set
- * outside tests; nh_reset clears the failure. */
-int nh_test_pair_wait(nh_sim *s, uint32_t spin_limit, uint32_t mute_wave0);
+/* Test hook for the give-up paths of the shapes that wait inside a launch
+ * (no counterpart in the reference): spin_limit = polls before a waiting
+ * wave gives up (0 = the default).  mute_wave0 = 1: in the paired shape wave
+ * 0 starts each launch 1 ms late, as a wave that is not resident would, so
+ * the waves it feeds give up waiting and recompute its first step
+ * themselves, and the results are unchanged; in the persistent shape wave 0
+ * withholds its published spike words.  Persistent shape: the launch ends early and the
+ * next synchronising call reports NH_ERR_HIP (nh_reset clears it).  The
+ * hook synchronises first; *fallbacks (may be null) gets the number of
+ * synchronisations so far that found a paired launch's wait given up.
+ * Never set outside tests. */
+int nh_test_pair_wait(nh_sim *s, uint32_t spin_limit, uint32_t mute_wave0, uint64_t *fallbacks);
 
 /* Measurement helpers (additions, no counterpart in the reference).
  * nh_probe_copy: copy `bytes` between two device buffers with a plain

@@ -117,7 +117,7 @@ def lib():
         "nh_profile_steps": (i32, [vp, u64, P(f64), P(u64), P(f64), P(u64)]),
         "nh_autotune": (i32, [vp, u32, C.c_char_p, u32]),
         "nh_launch_shape": (i32, [vp, C.POINTER(C.c_int32)]),
-        "nh_test_pair_wait": (i32, [vp, u32, u32]),
+        "nh_test_pair_wait": (i32, [vp, u32, u32, P(u64)]),
         "nh_probe_copy": (i32, [u64, u32, u32, P(f64)]),
         "nh_probe_launch": (i32, [u32, u32, P(f64)]),
         "nh_batch_autotune": (i32, [vp, u32, C.c_char_p, u32]),

@@ -871,18 +871,27 @@ bool Arena::pair_usable() {
     return pair_w8;
 }
 
-// A paired launch whose bounded wait ran out leaves the state invalid:
-// report it (once) at the next synchronisation.
+// A wave whose bounded wait ran out raises the device flag.  In a paired
+// launch the wave then recomputed the sources it waited for (results
+// unchanged): the give-up is counted, not reported.  A persistent launch
+// cannot recompute (its sources' state lives in registers), so there the
+// state is invalid: reported (once) at the next synchronisation.
 int Arena::check_pair_abort() {
     if (!pair_launched || !d_pair_abort.p) return NH_OK;
     pair_launched = false;
+    const bool persist = persist_launched;
+    persist_launched = false;
     // the host mirror (mapped memory, written by the wave that gave up) is
     // read directly: no copy on the common path
     if (!h_pair_abort.get()) return NH_OK;
     h_pair_abort.clear();
     HIPCHK(hipMemsetAsync(d_pair_abort.p, 0, 4, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    return set_error(NH_ERR_HIP, "paired-step launch: a wave's wait for its delay-1 sources ran out "
+    if (!persist) {
+        ++pair_fallbacks;
+        return NH_OK;
+    }
+    return set_error(NH_ERR_HIP, "persistent launch: a wave's wait for its sources ran out "
                                  "(workgroups not co-resident); the state is invalid, reset the simulation");
 }
 
@@ -974,6 +983,7 @@ int Arena::profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &
             if ((rc = launch_persistent(t, (uint32_t)k))) break;
             (void)hipEventRecord(ev[1], stream);
             pair_launched = true;
+            persist_launched = true;
             hipError_t e = hipStreamSynchronize(stream);
             if (e != hipSuccess) {
                 rc = set_error(NH_ERR_HIP, std::string("profile: ") + hipGetErrorString(e));
@@ -1141,6 +1151,7 @@ int Arena::run(uint64_t n_steps) {
             }
             TRY(launch_persistent(t, (uint32_t)k));
             pair_launched = true;
+            persist_launched = true;
             t += k;
             n_steps -= k;
         }
@@ -1264,6 +1275,7 @@ int Arena::reset() {
     }
     HIPCHK(hipStreamSynchronize(stream));
     pair_launched = false;
+    persist_launched = false;
     t = 0;
     rast_step0 = 0;
     h_rast_cnt.clear();

@@ -226,6 +226,8 @@ class Arena {
     PinnedWord h_pair_abort;   // host mirror of d_pair_abort
     bool pair_tables = false;
     bool pair_launched = false;          // paired launches since the last abort check
+    bool persist_launched = false;       // of them, persistent launches
+    uint64_t pair_fallbacks = 0;         // synchronisations that found a paired wait given up
     uint32_t pair_cap_block = 0, pair_cap_waves = 0, pair_cap8_waves = 0;
     bool pair_w8 = false;                // the next paired launches use the 8-wave build
     // test hook (failure handling): poll limit of the pair wait (0 = the

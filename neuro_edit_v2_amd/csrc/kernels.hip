@@ -115,6 +115,10 @@ __device__ __forceinline__ void raise_abort(const StepArgs &a) {
     if (a.pair_abort_host) __hip_atomic_store(a.pair_abort_host, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// s_waitcnt immediate for vmcnt(0) with the other counters left alone
+// (gfx9 encoding: vmcnt[3:0] and [15:14], expcnt[6:4], lgkmcnt[11:8]).
+constexpr uint32_t kWaitVmcnt0 = 0x0F70;
+
 constexpr uint32_t kPushUnroll = 4;
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;  // batch slot without an entry
 
@@ -806,21 +810,72 @@ uint32_t binned_lds_bytes(uint32_t entries, uint32_t block) { return entries * 8
 // pair_abort (the host reports the launch as failed) and moves on, and
 // every other waiting wave sees the flag and stops waiting.
 
-template <typename RT>
-__device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t ent, RT w,
-                                        uint32_t tag, RT *acc_w, uint32_t lane) {
+// Step t0 of the launch's first step for one delay-1 source neuron, when
+// its wave has not published its word in time (a wave that is not resident
+// at all, when another process holds compute units for longer than the
+// wait): the source's update recomputed from its inputs, which the launch
+// does not change before every wave has published -- x, s and the clear of
+// ring slot t0 are the launch's last stores, issued after the wave's own
+// publication has completed (see pair_steps) -- and ring slot t0 receives
+// no deliveries during the launch; the caller polls the word again after
+// the inputs were read, so a source that started meanwhile cannot have
+// overwritten them unnoticed.  The same arithmetic as the source's own
+// update (same noise key, parameter row and order), so the pulled sum, and
+// with the int ring the whole raster, stay exact.  Nothing is written for
+// the source.
+template <int RING, int PMODE, int MMODE, bool LEAN>
+__device__ __forceinline__ bool source_fired(const StepArgs &a, const NetRec &nr, uint32_t src, uint64_t tl0,
+                                          uint32_t slot0, uint32_t k, bool stim0, const uint8_t *prow_p) {
+    using RT = typename Ring<RING>::T;
+    const uint32_t local = src - nr.base;
+    if (!(local < nr.n && nr.active && (nr.t_len == 0 || tl0 < nr.t_len))) return false;
+    const RT rv = ((const RT *)a.ring)[(size_t)slot0 * a.n_total + src];
+    float s = a.s[src];
+    float2 xs = a.x[src];
+    const float ext = a.ext ? a.ext[src] : 0.0f;
+    const uint32_t prow = (PMODE != 1 && prow_p) ? prow_p[src] : 0u;
+    return neuron_update<RING, PMODE, MMODE, false, LEAN, false>(a, nr, src, local, tl0, slot0, k, stim0, prow,
+                                                                 rv, ext, Noise{false, 0.0f}, Params{}, s, xs);
+}
+
+// One 64-entry round of the wave's delay-1 in-table: wait for each entry's
+// source word (tag `tag`), then add the weight of every source that fired.
+// Past pair_spins polls (or once another wave has given up) the entries
+// still unpublished are recomputed (source_fired) instead of waited for,
+// and the give-up is noted in the abort flag (a count for the host, not an
+// error).
+template <int RING, int PMODE, int MMODE, bool LEAN, typename RT>
+__device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t ent, RT w, uint32_t tag,
+                                        RT *acc_w, uint32_t lane, const NetRec &nr, uint64_t tl0,
+                                        uint32_t slot0, uint32_t k, bool stim0, const uint8_t *prow_p) {
     const uint32_t src = ent & 0x03FFFFFFu;
     const uint64_t *pw = a.pub + 2u * (src >> 6) + ((src >> 5) & 1u);
     uint64_t v = 0;
+    bool fired = false;
     for (uint32_t spins = 0;; ++spins) {
         if (has) v = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const bool ok = !has || (uint32_t)(v >> 32) == tag;
         if (__ballot(!ok) == 0ull) break;
-        if (spins >= a.pair_spins ||
-            ((spins & 63u) == 63u &&
-             __hip_atomic_load(a.pair_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            if (lane == 0)
-                raise_abort(a);
+        if (__builtin_expect(spins >= a.pair_spins ||
+                                 ((spins & 63u) == 63u &&
+                                  __hip_atomic_load(a.pair_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)),
+                             0)) {
+            if (lane == 0) raise_abort(a);
+            if (!ok) {
+                fired = source_fired<RING, PMODE, MMODE, LEAN>(a, nr, src, tl0, slot0, k, stim0, prow_p);
+                // Poll once more after the inputs were read: still
+                // unpublished, the source had not stored over them when
+                // they were read (its stores follow its published word);
+                // published meanwhile, its word is used instead.
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                v = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(v >> 32) == tag) fired = ((uint32_t)v >> (src & 31u)) & 1u;
+            } else {
+                fired = ((uint32_t)v >> (src & 31u)) & 1u;
+            }
+            if (has && fired) atomicAdd(&acc_w[ent >> 26], w);
             return;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -847,6 +902,13 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     const uint32_t wave = blockIdx.x * wpb + uniform(threadIdx.x >> 6);
     if (wave >= words) return;
     const uint32_t i = wave * kWave + lane;
+    // test hook (nh_test_pair_wait): wave 0 starts late, as a wave that is
+    // not resident would, so the waves it feeds give up waiting first;
+    // bounded (1 ms on the 100 MHz real-time counter)
+    if (a.pair_mute && wave == 0u) {
+        const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
+        while (__builtin_amdgcn_s_memrealtime() - t_start < 100000ull) __builtin_amdgcn_s_sleep(127);
+    }
     // ---- every load of the launch up front (the state first, then the
     // network record's dependent loads)
     RT *ring = (RT *)ring_p;
@@ -895,7 +957,7 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
                                                                   s, xs);
     const uint64_t m0 = __ballot(sp0);
     const uint32_t tag = (uint32_t)t0 + 1u;   // never 0 before 2^32 steps: 0 is "unwritten"
-    if (lane < 2u && !(a.pair_mute && wave == 0)) {
+    if (lane < 2u) {
         const uint32_t half = lane ? (uint32_t)(m0 >> 32) : (uint32_t)m0;
         __hip_atomic_store(&a.pub[2u * wave + lane], ((uint64_t)tag << 32) | half, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -917,16 +979,21 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     // ---- pull the delay-1 arrivals of t0 into this wave's neurons
     RT *acc_w = acc + (threadIdx.x & ~(uint32_t)(kWave - 1));
     acc_w[lane] = (RT)0;
-    pull_d1<RT>(a, has0, ent0, w0, tag, acc_w, lane);
-    for (uint32_t b = pl_lo + kWave; b < pl_hi; b += kWave) {
-        const bool has = b + lane < pl_hi;
-        uint32_t ent = 0;
-        RT w = (RT)0;
+    // (one call site: the round's give-up path holds a whole neuron update)
+    bool has = has0;
+    uint32_t ent = ent0;
+    RT w = w0;
+    for (uint32_t b = pl_lo;;) {
+        pull_d1<RING, PMODE, MMODE, LEAN>(a, has, ent, w, tag, acc_w, lane, nr, tl0, slot0, k, stim0, prow_p);
+        b += kWave;
+        if (b >= pl_hi) break;
+        has = b + lane < pl_hi;
+        ent = 0;
+        w = (RT)0;
         if (has) {
             ent = a.pl_ent[b + lane];
             w = pl_w[b + lane];
         }
-        pull_d1<RT>(a, has, ent, w, tag, acc_w, lane);
     }
     __builtin_amdgcn_wave_barrier();
     const RT rv1p = rv1 + acc_w[lane];
@@ -939,7 +1006,13 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     const uint64_t m1 = __ballot(sp1);
     if (m1) push_wave<RING, PACKED, false>(a, m1, row_lo, row_hi, slot1, lane);
 
-    // ---- stores of both steps
+    // ---- stores of both steps.  They overwrite this launch's inputs, which
+    // a waiting wave may read to recompute this wave's step t0
+    // (source_fired) while this wave's word is unpublished to it: so they
+    // are issued only after the publication has completed.  A wave that
+    // polled has waited for all its earlier memory operations already (the
+    // poll's wait); one with an empty in-table waits here.
+    if (pl_lo == pl_hi) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     if (act0) ring[r0] = (RT)0;
     if (act1) ring[r1] = (RT)0;
     if (act0 || act1) {

@@ -165,7 +165,8 @@ struct StepArgs {
     gptr<uint32_t> pair_abort;
     gptr<uint32_t> pair_abort_host;   // the same flag in mapped host memory (read by the host without a copy)
     uint32_t pair_spins;         // polls before a wave gives up waiting
-    uint32_t pair_mute;          // test only: 1 = wave 0 never publishes
+    uint32_t pair_mute;          // test only: 1 = wave 0 starts late (paired) or
+                                 // never publishes (persistent)
     // persistent kernel (many steps per launch): steps of the launch; the
     // spike words it publishes, kPubDepth steps deep (pub3[(t % 3)][2w +
     // half], tag t + 1); per step residue r = t % 4 and group g of

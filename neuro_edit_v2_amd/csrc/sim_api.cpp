@@ -218,9 +218,10 @@ int nh_profile_steps(nh_sim *s, uint64_t n_steps, double *main_ms, uint64_t *mai
     return NH_OK;
 }
 
-int nh_test_pair_wait(nh_sim *s, uint32_t spin_limit, uint32_t mute_wave0) {
+int nh_test_pair_wait(nh_sim *s, uint32_t spin_limit, uint32_t mute_wave0, uint64_t *fallbacks) {
     CHECK_SIM(s);
     TRY(s->ar.sync());
+    if (fallbacks) *fallbacks = s->ar.pair_fallbacks;
     s->ar.pair_spins = spin_limit;
     s->ar.pair_mute = mute_wave0 ? 1u : 0u;
     s->ar.graph_dirty = true;  // both ride in the captured kernel arguments

@@ -223,7 +223,7 @@ extern "C" {
     ) -> c_int;
     pub fn nh_autotune(s: *mut NhSim, trial_steps: u32, report: *mut c_char, report_cap: u32) -> c_int;
     pub fn nh_launch_shape(s: *mut NhSim, shape: *mut i32) -> c_int;
-    pub fn nh_test_pair_wait(s: *mut NhSim, spin_limit: u32, mute_wave0: u32) -> c_int;
+    pub fn nh_test_pair_wait(s: *mut NhSim, spin_limit: u32, mute_wave0: u32, fallbacks: *mut u64) -> c_int;
     pub fn nh_probe_copy(bytes: u64, reps: u32, lane_bytes: u32, ms: *mut f64) -> c_int;
     pub fn nh_probe_launch(blocks: u32, launches: u32, us: *mut f64) -> c_int;
 

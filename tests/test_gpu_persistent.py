@@ -164,13 +164,13 @@ def test_persistent_wait_gives_up_and_reports():
     kw = dict(CFG0, int_weights=True)
     sim = Sim(net, SimConfig(raster_steps=200, step_shape=3, **kw))
     assert sim.launch_shape() == 3
-    L.check(L.lib().nh_test_pair_wait(sim._h, 64, 1))
+    L.check(L.lib().nh_test_pair_wait(sim._h, 64, 1, None))
     print("case: persistent wait, wave 0 muted, 64 polls", flush=True)
     sim.run_steps(20)
     with pytest.raises(L.NHError) as e:
         sim.synchronize()
     assert e.value.code == L.NH_ERR_HIP and "reset" in str(e.value)
-    L.check(L.lib().nh_test_pair_wait(sim._h, 0, 0))
+    L.check(L.lib().nh_test_pair_wait(sim._h, 0, 0, None))
     sim.reset_state()
     sim.run_steps(200)
     _, cnt, ids = sim.read_spikes()
