@@ -105,6 +105,37 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// A wave-uniform value that no kernel writes while a step kernel runs (step
+// counter, per-wave table bounds, network records), read through the
+// constant address space so that it is a scalar load whatever precedes it
+// (the compiler proves a plain load unclobbered only up to the first
+// instruction that may write memory): a wave's vector loads complete in
+// order, so a vector load of it issued behind the lanes' state loads could
+// be used only once they had all returned.
+// Wait here for the load of v (an empty asm that reads it).  Where a value
+// loaded long before is first read inside a loop, the wait the compiler
+// places there also covers whatever the loop's other paths leave
+// outstanding (in the paired kernel: the spike word just published, loads
+// just issued); read once here, the loop needs no wait for it.
+__device__ __forceinline__ void ready(uint32_t v) { asm volatile("" ::"v"(v)); }
+
+template <typename T>
+__device__ __forceinline__ T sload(gptr<const T> p) {
+    static_assert(sizeof(T) % 4 == 0, "scalar loads read whole words");
+    if constexpr (sizeof(T) <= 8) {
+        return *(const __attribute__((address_space(4))) T *)p;
+    } else {
+        // a record: word by word (one scalar load of each run of words)
+        const __attribute__((address_space(4))) uint32_t *q = (const __attribute__((address_space(4))) uint32_t *)p;
+        uint32_t w[sizeof(T) / 4];
+#pragma unroll
+        for (uint32_t j = 0; j < sizeof(T) / 4; ++j) w[j] = q[j];
+        T r;
+        __builtin_memcpy(&r, w, sizeof(T));
+        return r;
+    }
+}
+
 // Entries pushed per lane before the first atomic of a batch: every load of
 // a batch is in flight at once, so a batch costs one memory round trip.
 // A wave that gives up a bounded wait raises the abort flag: in device
@@ -140,45 +171,68 @@ __device__ __forceinline__ uint64_t ring_index(const StepArgs &a, uint32_t sy, u
     return (uint64_t)sl * a.n_total + tgt;
 }
 
-// Load the entries j[0..kPushUnroll) (kNoEntry = none) and deliver them:
-// all loads first (unconditional: an empty slot reads entry 0), then the
-// atomics.  With a ring below 2 GB the atomics are buffer atomics whose
-// empty slots carry an out-of-range offset, so the batch has no branch
-// per slot (branches there make the compiler wait for each atomic before
-// the next); otherwise each slot's atomic is guarded.  SKIP1 drops the
-// delay-1 entries (the paired-step kernel pulls those instead).
-template <int RING, bool PACKED, bool SKIP1 = false>
-__device__ __forceinline__ void deliver_batch(const StepArgs &a, const uint32_t (&j)[kPushUnroll],
-                                              uint32_t slot0) {
-    using RT = typename Ring<RING>::T;
-    const RT *w = (const RT *)a.w;
+// One batch of a push: up to kPushUnroll entries per lane (kNoEntry = none),
+// their synapse words and weights once loaded.
+template <typename RT>
+struct PushBatch {
+    uint32_t j[kPushUnroll];
     uint32_t sy[kPushUnroll], d16[kPushUnroll];
     RT wv[kPushUnroll];
+};
+
+// Load a batch's entries (unconditional: an empty slot reads entry 0).
+template <int RING, bool PACKED>
+__device__ __forceinline__ void load_batch(const StepArgs &a, PushBatch<typename Ring<RING>::T> &pb) {
+    using RT = typename Ring<RING>::T;
+    const RT *w = (const RT *)a.w;
 #pragma unroll
     for (uint32_t r = 0; r < kPushUnroll; ++r) {
-        const uint32_t jr = j[r] != kNoEntry ? j[r] : 0u;
-        sy[r] = a.syn[jr];
-        wv[r] = w[jr];
-        d16[r] = PACKED ? 0u : a.delay16[jr];
+        const uint32_t jr = pb.j[r] != kNoEntry ? pb.j[r] : 0u;
+        pb.sy[r] = a.syn[jr];
+        pb.wv[r] = w[jr];
+        pb.d16[r] = PACKED ? 0u : a.delay16[jr];
     }
+}
+
+// Deliver a loaded batch.  With a ring below 2 GB the atomics are buffer
+// atomics whose empty slots carry an out-of-range offset, so the batch has
+// no branch per slot (branches there make the compiler wait for each atomic
+// before the next); otherwise each slot's atomic is guarded.  SKIP1 drops
+// the delay-1 entries (the paired-step kernel pulls those instead).
+template <int RING, bool PACKED, bool SKIP1 = false>
+__device__ __forceinline__ void send_batch(const StepArgs &a, const PushBatch<typename Ring<RING>::T> &pb,
+                                           uint32_t slot0) {
+    using RT = typename Ring<RING>::T;
     bool keep[kPushUnroll];
 #pragma unroll
     for (uint32_t r = 0; r < kPushUnroll; ++r)
-        keep[r] = j[r] != kNoEntry && (!SKIP1 || (PACKED ? sy[r] >> 27 : d16[r]) != 1u);
+        keep[r] = pb.j[r] != kNoEntry && (!SKIP1 || (PACKED ? pb.sy[r] >> 27 : pb.d16[r]) != 1u);
     if (a.ring_buf) {
         const i32x4 rs = buffer_rsrc(a.ring, a.ring_bytes);
 #pragma unroll
         for (uint32_t r = 0; r < kPushUnroll; ++r) {
             // ring below 2 GB: the byte offset fits in 31 bits
-            const uint32_t off = (uint32_t)ring_index<PACKED>(a, sy[r], d16[r], slot0) * 4u;
-            Ring<RING>::add_buf(rs, keep[r] ? off : kBufDrop, wv[r]);
+            const uint32_t off = (uint32_t)ring_index<PACKED>(a, pb.sy[r], pb.d16[r], slot0) * 4u;
+            Ring<RING>::add_buf(rs, keep[r] ? off : kBufDrop, pb.wv[r]);
         }
     } else {
 #pragma unroll
         for (uint32_t r = 0; r < kPushUnroll; ++r)
             if (keep[r])
-                Ring<RING>::add((RT *)a.ring + ring_index<PACKED>(a, sy[r], d16[r], slot0), wv[r]);
+                Ring<RING>::add((RT *)a.ring + ring_index<PACKED>(a, pb.sy[r], pb.d16[r], slot0), pb.wv[r]);
     }
+}
+
+// Load the entries j[0..kPushUnroll) and deliver them: all loads first,
+// then the atomics.
+template <int RING, bool PACKED, bool SKIP1 = false>
+__device__ __forceinline__ void deliver_batch(const StepArgs &a, const uint32_t (&j)[kPushUnroll],
+                                              uint32_t slot0) {
+    PushBatch<typename Ring<RING>::T> pb;
+#pragma unroll
+    for (uint32_t r = 0; r < kPushUnroll; ++r) pb.j[r] = j[r];
+    load_batch<RING, PACKED>(a, pb);
+    send_batch<RING, PACKED, SKIP1>(a, pb, slot0);
 }
 
 // Push one spike's out-synapse row into the ring; lanes stride the row
@@ -198,49 +252,94 @@ __device__ __forceinline__ void push_row(const StepArgs &a, uint32_t lo, uint32_
     }
 }
 
-// Push the rows of every spiking lane of a wave as one list: the rows are
-// concatenated in lane order and the wave walks the list kPushUnroll*64
-// entries at a time, so a wave whose spikes own few synapses pays one load
-// round trip in total instead of one per spike and per 64 entries.
-template <int RING, bool PACKED, bool SKIP1 = false>
-__device__ __forceinline__ void push_wave(const StepArgs &a, uint64_t mask, uint32_t row_lo,
-                                          uint32_t row_hi, uint32_t slot0, uint32_t lane) {
-    constexpr uint32_t kSpan = kPushUnroll * kWave;
+// A wave's push as one list: the rows of its spiking lanes concatenated in
+// lane order, walked kPushUnroll*64 entries at a time, so a wave whose
+// spikes own few synapses pays one load round trip in total instead of one
+// per spike and per 64 entries.  `live`: spikes whose rows are not fully
+// issued (always a suffix of the spikes in lane order); `live_start`: list
+// offset of the first.
+struct PushList {
+    uint64_t live;
+    uint32_t live_start, total;
+};
+
+__device__ __forceinline__ PushList push_list(uint64_t mask, uint32_t row_lo, uint32_t row_hi) {
     uint32_t total = 0;
     for (uint64_t m = mask; m;) {
         const uint32_t b = (uint32_t)__builtin_ctzll(m);
         m &= m - 1;
         total += __builtin_amdgcn_readlane(row_hi, b) - __builtin_amdgcn_readlane(row_lo, b);
     }
-    // `live`: spikes whose rows are not fully issued (always a suffix of
-    // the spikes in lane order); `live_start`: list offset of the first
-    uint64_t live = mask;
-    uint32_t live_start = 0;
-    for (uint32_t base = 0; base < total; base += kSpan) {
-        const uint32_t end = base + kSpan;
-        uint32_t j[kPushUnroll];
+    return PushList{mask, 0u, total};
+}
+
+// The entries of list positions [base, base + kPushUnroll*64) into j.
+__device__ __forceinline__ void list_batch(PushList &pl, uint32_t base, uint32_t row_lo, uint32_t row_hi,
+                                           uint32_t lane, uint32_t (&j)[kPushUnroll]) {
+    const uint32_t end = base + kPushUnroll * kWave;
 #pragma unroll
-        for (uint32_t r = 0; r < kPushUnroll; ++r) j[r] = kNoEntry;
-        uint32_t s0 = live_start;
-        for (uint64_t m = live; m;) {
-            if (s0 >= end) break;
-            const uint32_t b = (uint32_t)__builtin_ctzll(m);
-            m &= m - 1;
-            const uint32_t lo = __builtin_amdgcn_readlane(row_lo, b);
-            const uint32_t d = __builtin_amdgcn_readlane(row_hi, b) - lo;
+    for (uint32_t r = 0; r < kPushUnroll; ++r) j[r] = kNoEntry;
+    uint32_t s0 = pl.live_start;
+    for (uint64_t m = pl.live; m;) {
+        if (s0 >= end) break;
+        const uint32_t b = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        const uint32_t lo = __builtin_amdgcn_readlane(row_lo, b);
+        const uint32_t d = __builtin_amdgcn_readlane(row_hi, b) - lo;
 #pragma unroll
-            for (uint32_t r = 0; r < kPushUnroll; ++r) {
-                const uint32_t u = base + r * kWave + lane - s0;
-                if (u < d) j[r] = lo + u;
-            }
-            if (s0 + d <= end) {
-                live &= ~(1ull << b);
-                live_start = s0 + d;
-            }
-            s0 += d;
+        for (uint32_t r = 0; r < kPushUnroll; ++r) {
+            const uint32_t u = base + r * kWave + lane - s0;
+            if (u < d) j[r] = lo + u;
         }
+        if (s0 + d <= end) {
+            pl.live &= ~(1ull << b);
+            pl.live_start = s0 + d;
+        }
+        s0 += d;
+    }
+}
+
+// Push the list's batches from position `from` on.
+template <int RING, bool PACKED, bool SKIP1 = false>
+__device__ __forceinline__ void push_list_from(const StepArgs &a, PushList &pl, uint32_t from,
+                                               uint32_t row_lo, uint32_t row_hi, uint32_t slot0,
+                                               uint32_t lane) {
+    for (uint32_t base = from; base < pl.total; base += kPushUnroll * kWave) {
+        uint32_t j[kPushUnroll];
+        list_batch(pl, base, row_lo, row_hi, lane, j);
         deliver_batch<RING, PACKED, SKIP1>(a, j, slot0);
     }
+}
+
+// The list of a wave's push and its first batch's entries; a wave with one
+// spike (the common case) takes them without walking the mask.
+__device__ __forceinline__ PushList push_first(uint64_t mask, uint32_t row_lo, uint32_t row_hi, uint32_t lane,
+                                               uint32_t (&j)[kPushUnroll]) {
+    if ((mask & (mask - 1u)) == 0ull) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(mask);
+        const uint32_t lo = __builtin_amdgcn_readlane(row_lo, b), hi = __builtin_amdgcn_readlane(row_hi, b);
+#pragma unroll
+        for (uint32_t r = 0; r < kPushUnroll; ++r) {
+            const uint32_t jr = lo + r * kWave + lane;
+            j[r] = jr < hi ? jr : kNoEntry;
+        }
+        const uint32_t d = hi - lo;
+        const bool done = d <= kPushUnroll * kWave;
+        return PushList{done ? 0ull : mask, done ? d : 0u, d};
+    }
+    PushList pl = push_list(mask, row_lo, row_hi);
+    list_batch(pl, 0u, row_lo, row_hi, lane, j);
+    return pl;
+}
+
+// Push the rows of every spiking lane of a wave.
+template <int RING, bool PACKED, bool SKIP1 = false>
+__device__ __forceinline__ void push_wave(const StepArgs &a, uint64_t mask, uint32_t row_lo,
+                                          uint32_t row_hi, uint32_t slot0, uint32_t lane) {
+    uint32_t j[kPushUnroll];
+    PushList pl = push_first(mask, row_lo, row_hi, lane, j);
+    deliver_batch<RING, PACKED, SKIP1>(a, j, slot0);
+    push_list_from<RING, PACKED, SKIP1>(a, pl, kPushUnroll * kWave, row_lo, row_hi, slot0, lane);
 }
 
 // Custom-tau synapses of one spike: arrival marks in the per-synapse ring
@@ -539,7 +638,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, fl
         nr = a.net0;
     } else {
         net = uniform((uint32_t)a.wave_net[wave]);
-        nr = a.nets[net];
+        nr = sload(a.nets + net);
     }
     const float ext = a.ext ? a.ext[i] : 0.0f;
     // CSR row bounds: loaded with the state, except in the lean variant (one
@@ -554,14 +653,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, fl
     // (the step is read from the device counter even where the host knows
     // it: given by value, it measured slower -- the noise hashes then get
     // scheduled among the first loads)
-    const uint64_t t = *a.t_dev + k;
-    // raster window origin, read here (before any store of this kernel) so
-    // it is a scalar load issued with the others, not a dependent vector
-    // load after the state stores
-    const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
+    const uint64_t t = sload(a.t_dev) + k;
+    // raster window origin (a scalar load issued with the others)
+    const uint64_t rast_origin = a.rast_bits ? sload<uint64_t>(a.rast_ctl + 1) : 0;
     // does this network have a nonzero input this step?  (scalar load,
     // issued with the others)
-    const bool stim_on = !LEAN && a.stim_lo && a.stim_nz[(size_t)k * a.stim_groups + net] != 0u;
+    const bool stim_on = !LEAN && a.stim_lo && sload(a.stim_nz + (size_t)k * a.stim_groups + net) != 0u;
 
     const uint32_t local = i - nr.base;
     const uint64_t tl = t - (uint64_t)nr.t0;
@@ -673,7 +770,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
             nr = a.net0;
         } else {
             net = uniform((uint32_t)a.wave_net[wave]);
-            nr = a.nets[net];
+            nr = sload(a.nets + net);
         }
         RT *ring = (RT *)a.ring;
         const size_t ri = (size_t)slot * a.n_total + i;
@@ -684,9 +781,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
         const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
         uint32_t prow = 0;
         if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
-        const uint64_t t = *a.t_dev + a.k;
-        const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
-        const bool stim_on = a.stim_lo && a.stim_nz[(size_t)a.k * a.stim_groups + net] != 0u;
+        const uint64_t t = sload(a.t_dev) + a.k;
+        const uint64_t rast_origin = a.rast_bits ? sload<uint64_t>(a.rast_ctl + 1) : 0;
+        const bool stim_on = a.stim_lo && sload(a.stim_nz + (size_t)a.k * a.stim_groups + net) != 0u;
         const uint32_t local = i - nr.base;
         const uint64_t tl = t - (uint64_t)nr.t0;
         const bool act = local < nr.n && nr.active && (nr.t_len == 0 || tl < nr.t_len);
@@ -888,7 +985,7 @@ __device__ __forceinline__ void pull_d1(const StepArgs &a, bool has, uint32_t en
 // (-amdgpu-kernarg-preload-count): the state and ring loads issue at once,
 // without waiting for a load of the argument block (whose other fields are
 // needed only after those loads return).
-template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
+template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN, bool SPEC>
 __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p, uint32_t slots,
                                            uint32_t n_total, uint32_t words_wpb, uint32_t k,
                                            const uint8_t *prow_p, const StepArgs &a) {
@@ -902,13 +999,6 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     const uint32_t wave = blockIdx.x * wpb + uniform(threadIdx.x >> 6);
     if (wave >= words) return;
     const uint32_t i = wave * kWave + lane;
-    // test hook (nh_test_pair_wait): wave 0 starts late, as a wave that is
-    // not resident would, so the waves it feeds give up waiting first;
-    // bounded (1 ms on the 100 MHz real-time counter)
-    if (a.pair_mute && wave == 0u) {
-        const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
-        while (__builtin_amdgcn_s_memrealtime() - t_start < 100000ull) __builtin_amdgcn_s_sleep(127);
-    }
     // ---- every load of the launch up front (the state first, then the
     // network record's dependent loads)
     RT *ring = (RT *)ring_p;
@@ -925,56 +1015,92 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
         nr = a.net0;
     } else {
         net = uniform((uint32_t)a.wave_net[wave]);
-        nr = a.nets[net];
+        nr = sload(a.nets + net);
     }
     const float ext = a.ext ? a.ext[i] : 0.0f;
     const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
-    const uint64_t t0 = *a.t_dev + k;
-    const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
-    const bool stim0 = !LEAN && a.stim_lo && a.stim_nz[(size_t)k * a.stim_groups + net] != 0u;
-    const bool stim1 = !LEAN && a.stim_lo && a.stim_nz[(size_t)(k + 1) * a.stim_groups + net] != 0u;
-    // this wave's delay-1 in-table; the first 64 entries are loaded here
-    const uint32_t pl_lo = a.pl_ptr[wave], pl_hi = a.pl_hi[wave];
+    const uint64_t t0 = sload(a.t_dev) + k;
+    const uint64_t rast_origin = a.rast_bits ? sload<uint64_t>(a.rast_ctl + 1) : 0;
+    const bool stim0 = !LEAN && a.stim_lo && sload(a.stim_nz + (size_t)k * a.stim_groups + net) != 0u;
+    const bool stim1 = !LEAN && a.stim_lo && sload(a.stim_nz + (size_t)(k + 1) * a.stim_groups + net) != 0u;
+    // this wave's delay-1 in-table; the first 64 entries are loaded here.
+    // Lanes past the table's end load entry 0 (the table always holds one)
+    // and never use it: with no branch around the loads, the compiler knows
+    // the order of every load in flight and waits for each exactly.
+    const uint32_t pl_lo = sload(a.pl_ptr + wave), pl_hi = sload(a.pl_hi + wave);
     const RT *pl_w = (const RT *)a.pl_w;
     const bool has0 = pl_lo + lane < pl_hi;
-    uint32_t ent0 = 0;
-    RT w0 = (RT)0;
-    if (has0) {
-        ent0 = a.pl_ent[pl_lo + lane];
-        w0 = pl_w[pl_lo + lane];
+    const uint32_t e0 = has0 ? pl_lo + lane : 0u;
+    const uint32_t ent0 = a.pl_ent[e0];
+    const RT w0 = pl_w[e0];
+
+    // test hook (nh_test_pair_wait): wave 0 steps late, as a wave that is
+    // not resident would, so the waves it feeds give up waiting first;
+    // bounded (1 ms on the 100 MHz real-time counter)
+    if (wave == 0u && a.pair_mute) {
+        const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
+        while (__builtin_amdgcn_s_memrealtime() - t_start < 100000ull) __builtin_amdgcn_s_sleep(127);
     }
 
     // ---- step t0
     const uint32_t local = i - nr.base;
     const uint64_t tl0 = t0 - (uint64_t)nr.t0;
+    const uint64_t t1 = t0 + 1u, tl1 = tl0 + 1u;
     const bool act0 = local < nr.n && nr.active && (nr.t_len == 0 || tl0 < nr.t_len);
+    const bool act1 = local < nr.n && nr.active && (nr.t_len == 0 || tl1 < nr.t_len);
+    // The noise of both steps depends only on (neuron, step, seed): hashed
+    // for every lane here, while the state loads are in flight (a lane that
+    // does not step ignores its draws).
+    const Noise nz0 = noise_terms(nr, local, tl0);
+    const Noise nz1 = noise_terms(nr, local, tl1);
     bool sp0 = false;
     // the parameter row serves both steps
     const Params pp = select_params<PMODE, MMODE, true>(a, i, prow);
     if (act0)
-        sp0 = neuron_update<RING, PMODE, MMODE, false, LEAN, true>(a, nr, i, local, tl0, slot0, k, stim0,
-                                                                  prow, rv0, ext, Noise{false, 0.0f}, pp,
-                                                                  s, xs);
+        sp0 = neuron_update<RING, PMODE, MMODE, true, LEAN, true>(a, nr, i, local, tl0, slot0, k, stim0, prow,
+                                                                 rv0, ext, nz0, pp, s, xs);
     const uint64_t m0 = __ballot(sp0);
+    ready(row_lo);
+    ready(row_hi);
     const uint32_t tag = (uint32_t)t0 + 1u;   // never 0 before 2^32 steps: 0 is "unwritten"
     if (lane < 2u) {
         const uint32_t half = lane ? (uint32_t)(m0 >> 32) : (uint32_t)m0;
         __hip_atomic_store(&a.pub[2u * wave + lane], ((uint64_t)tag << 32) | half, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    // The push and stores of t0 go between the publication and the poll:
-    // another XCD sees a published word only about two memory round trips
-    // after it is stored, and this work fills that time (done after the
-    // poll it would add its own round trips to the end of the launch).
-    if (m0) push_wave<RING, PACKED, true>(a, m0, row_lo, row_hi, slot0, lane);
-    if (sp0) spike_marks<LEAN>(a, nr, i, t0, tl0);
-    if (lane == 0) spike_words(a, wave, t0, rast_origin, m0);
+    // The push of t0: its first batch's entries are loaded here, so that
+    // their round trip runs beside the poll's, and delivered after the
+    // pull.  A wave's memory operations complete in order, so atomics (or
+    // any access) issued before the poll would hold up its result by their
+    // own completion; the spike marks and words wait until after it too.
+    ready(ent0);
+    PushList pl0{0ull, 0u, 0u};
+    PushBatch<RT> pb0;
+    if (m0) {
+        pl0 = push_first(m0, row_lo, row_hi, lane, pb0.j);
+        load_batch<RING, PACKED>(a, pb0);
+    }
 
-    // step t1's noise does not depend on the pulled input: computed before
-    // the wait, it leaves the wait's end with less work behind it
-    const uint64_t t1 = t0 + 1u, tl1 = tl0 + 1u;
-    const bool act1 = local < nr.n && nr.active && (nr.t_len == 0 || tl1 < nr.t_len);
-    const Noise nz1 = act1 ? noise_terms(nr, local, tl1) : Noise{false, 0.0f};
+    // SPEC: step t1 speculatively here, as if no delay-1 arrival
+    // came (a pulled sum of zero, the case of nearly every wave), and load
+    // the first batch of its push: kept if the wave's pulled sums are all
+    // zero (the same operations on the same values: rv1 + 0), redone after
+    // the pull otherwise.  The update only reads memory in a paired launch
+    // (no custom-tau traces), so a discarded one leaves nothing behind.
+    float s_sp = s;
+    float2 xs_sp = xs;
+    bool sp1_sp = false;
+    if (SPEC && act1)
+        sp1_sp = neuron_update<RING, PMODE, MMODE, true, LEAN, true>(a, nr, i, local, tl1, slot1, k + 1u,
+                                                                    stim1, prow, rv1 + (RT)0, ext, nz1, pp,
+                                                                    s_sp, xs_sp);
+    const uint64_t m1_sp = SPEC ? __ballot(sp1_sp) : 0ull;
+    PushList pl1{0ull, 0u, 0u};
+    PushBatch<RT> pb1;
+    if (SPEC && m1_sp) {
+        pl1 = push_first(m1_sp, row_lo, row_hi, lane, pb1.j);
+        load_batch<RING, PACKED>(a, pb1);
+    }
 
     // ---- pull the delay-1 arrivals of t0 into this wave's neurons
     RT *acc_w = acc + (threadIdx.x & ~(uint32_t)(kWave - 1));
@@ -988,23 +1114,43 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
         b += kWave;
         if (b >= pl_hi) break;
         has = b + lane < pl_hi;
-        ent = 0;
-        w = (RT)0;
-        if (has) {
-            ent = a.pl_ent[b + lane];
-            w = pl_w[b + lane];
-        }
+        const uint32_t e = has ? b + lane : 0u;
+        ent = a.pl_ent[e];
+        w = pl_w[e];
+        ready(ent);
     }
+    if (m0) {
+        send_batch<RING, PACKED, true>(a, pb0, slot0);
+        push_list_from<RING, PACKED, true>(a, pl0, kPushUnroll * kWave, row_lo, row_hi, slot0, lane);
+    }
+    if (sp0) spike_marks<LEAN>(a, nr, i, t0, tl0);
+    if (lane == 0) spike_words(a, wave, t0, rast_origin, m0);
     __builtin_amdgcn_wave_barrier();
-    const RT rv1p = rv1 + acc_w[lane];
+    const RT pulled = acc_w[lane];
 
     // ---- step t1
     bool sp1 = false;
-    if (act1)
-        sp1 = neuron_update<RING, PMODE, MMODE, true, LEAN, true>(a, nr, i, local, tl1, slot1, k + 1u,
-                                                                 stim1, prow, rv1p, ext, nz1, pp, s, xs);
-    const uint64_t m1 = __ballot(sp1);
-    if (m1) push_wave<RING, PACKED, false>(a, m1, row_lo, row_hi, slot1, lane);
+    uint64_t m1;
+    if (SPEC && __ballot(pulled != (RT)0) == 0ull) {
+        s = s_sp;
+        xs = xs_sp;
+        sp1 = sp1_sp;
+        m1 = m1_sp;
+    } else {
+        if (act1)
+            sp1 = neuron_update<RING, PMODE, MMODE, true, LEAN, true>(a, nr, i, local, tl1, slot1, k + 1u,
+                                                                     stim1, prow, rv1 + pulled, ext, nz1,
+                                                                     pp, s, xs);
+        m1 = __ballot(sp1);
+    }
+    if (m1) {
+        if (SPEC && m1 == m1_sp) {
+            send_batch<RING, PACKED, false>(a, pb1, slot1);
+            push_list_from<RING, PACKED, false>(a, pl1, kPushUnroll * kWave, row_lo, row_hi, slot1, lane);
+        } else {
+            push_wave<RING, PACKED, false>(a, m1, row_lo, row_hi, slot1, lane);
+        }
+    }
 
     // ---- stores of both steps.  They overwrite this launch's inputs, which
     // a waiting wave may read to recompute this wave's step t0
@@ -1027,7 +1173,8 @@ template <int RING, int PMODE, int MMODE, bool PACKED, bool LEAN>
 __global__ __launch_bounds__(kMaxBlock) void k_pair(void *ring, float *s, float2 *x, uint32_t slots,
                                                     uint32_t n_total, uint32_t words_wpb, uint32_t k,
                                                     const uint8_t *prow, ArgBlock<LEAN> av) {
-    pair_steps<RING, PMODE, MMODE, PACKED, LEAN>(ring, s, x, slots, n_total, words_wpb, k, prow, arg_block(av));
+    pair_steps<RING, PMODE, MMODE, PACKED, LEAN, LEAN>(ring, s, x, slots, n_total, words_wpb, k, prow,
+                                                       arg_block(av));
 }
 
 // The same kernel held to the register budget of 8 waves per SIMD (the
@@ -1040,7 +1187,8 @@ template <int RING, int PMODE, int MMODE, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pair8(
     void *ring, float *s, float2 *x, uint32_t slots, uint32_t n_total, uint32_t words_wpb, uint32_t k,
     const uint8_t *prow, ArgBlock<true> av) {
-    pair_steps<RING, PMODE, MMODE, PACKED, true>(ring, s, x, slots, n_total, words_wpb, k, prow, arg_block(av));
+    pair_steps<RING, PMODE, MMODE, PACKED, true, false>(ring, s, x, slots, n_total, words_wpb, k, prow,
+                                                        arg_block(av));
 }
 
 // ---------------------------------------------------------------- persistent
@@ -1178,9 +1326,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_run(StepArgs a) {
     const uint32_t row_lo = a.row_lo[i], row_hi = a.row_hi[i];
     uint32_t prow = 0;
     if (PMODE != 1 && (PMODE == 2 || a.n_krows > 1)) prow = a.prow[i];
-    const uint64_t t0 = *a.t_dev + a.k;
-    const uint64_t rast_origin = a.rast_bits ? a.rast_ctl[1] : 0;
-    const uint32_t pl_lo = a.pl_ptr[wave], pl_hi = a.pl_hi[wave];
+    const uint64_t t0 = sload(a.t_dev) + a.k;
+    const uint64_t rast_origin = a.rast_bits ? sload<uint64_t>(a.rast_ctl + 1) : 0;
+    const uint32_t pl_lo = sload(a.pl_ptr + wave), pl_hi = sload(a.pl_hi + wave);
     const RT *pl_w = (const RT *)a.pl_w;
     const bool has0 = pl_lo + lane < pl_hi;
     uint32_t ent0 = 0;
@@ -1318,7 +1466,7 @@ template <int RING, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) void k_scatter(StepArgs a) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + uniform(threadIdx.x >> 6);
-    const uint64_t t = *a.t_dev + a.k;
+    const uint64_t t = sload(a.t_dev) + a.k;
     const uint32_t L = (uint32_t)(t % 3);
     const uint32_t c = a.cnt[L];
     const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
@@ -1335,7 +1483,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_scatter(StepArgs a) {
 __global__ void k_watch(StepArgs a) {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= a.n_watch) return;
-    const uint64_t t = *a.t_dev + a.k;
+    const uint64_t t = sload(a.t_dev) + a.k;
     a.xrec[(size_t)(t % a.xrec_depth) * a.n_watch + w] = a.x[a.watch[w]];
 }
 
