@@ -129,6 +129,10 @@ class Arena {
     // Run n steps asynchronously (stimulus windows packed by the packer).
     int run(uint64_t n_steps);
     int sync();
+    // sync(), then the last run's event times into last_run_ms and
+    // last_tail_ms (read only when asked for: two event queries would
+    // otherwise lengthen every synchronisation)
+    int sync_timed();
     int upload_recs();
     static void fill_noise(NetRec &r, const nh_config &c);
     uint32_t neuron_blocks() const;
@@ -171,6 +175,13 @@ class Arena {
     uint32_t n_nets = 0, n_total = 0, D = 2, words = 0, bits_rows = 0;
     int ring_int = 0, pmode = 0, mmode = 0, packed = 1, fused = 0, use_graph = 0;
     uint64_t t = 0;            // arena steps done
+    // The step counter d_t as it stands once the enqueued work has run.  The
+    // step kernels read their step as d_t + k; direct launches without
+    // stimulus inputs give it as an offset from this value (no counter
+    // update per run), captured chunks and the rest bring d_t to their
+    // first step and advance it by their length.
+    uint64_t t_dev_val = 0;
+    bool capturing = false;    // launch_chunk records into a graph
     uint64_t n_syn = 0;
     std::vector<NetSlot> slots;
     std::vector<NetRec> recs;
@@ -284,6 +295,7 @@ class Arena {
     int profile(uint64_t n_steps, double &main_ms, uint64_t &main_n, double &aux_ms,
                 uint64_t &aux_n);
     int capture_graph();
+    int sync_t_dev(uint64_t to);
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
 

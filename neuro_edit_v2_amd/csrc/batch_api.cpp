@@ -135,7 +135,7 @@ int nh_batch_sync(nh_batch *b) {
 
 int nh_batch_last_run_ms(nh_batch *b, double *ms) {
     CHECK_B(b);
-    TRY(b->ar.sync());
+    TRY(b->ar.sync_timed());
     *ms = b->ar.last_run_ms;
     return NH_OK;
 }

@@ -119,6 +119,29 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
 // just issued); read once here, the loop needs no wait for it.
 __device__ __forceinline__ void ready(uint32_t v) { asm volatile("" ::"v"(v)); }
 
+// A step output written through to memory (WT): an agent-scope relaxed
+// atomic store of the same value (a global store with the write-through
+// bit).  The end of a launch then has few dirty L2 lines left to write back.
+// Measured: the paired kernel at 100k neurons 5.36 -> 5.00 us per launch,
+// the 256-network batch 26.4 -> 25.5 us per step; the lean one-step kernel
+// at 1M neurons, which streams, 9.0 -> 9.3 us, so it keeps plain stores
+// (WT false).  Nontemporal stores, which stay in L2, changed nothing.
+template <bool WT = true, typename T>
+__device__ __forceinline__ void store_wt(T *p, T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "word or double word");
+    if constexpr (!WT) {
+        *p = v;
+    } else if constexpr (sizeof(T) == 4) {
+        uint32_t b;
+        __builtin_memcpy(&b, &v, 4);
+        __hip_atomic_store((uint32_t *)p, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        uint64_t b;
+        __builtin_memcpy(&b, &v, 8);
+        __hip_atomic_store((uint64_t *)p, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T sload(gptr<const T> p) {
     static_assert(sizeof(T) % 4 == 0, "scalar loads read whole words");
@@ -692,9 +715,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, fl
     }
     // ---- this step's stores
     if (act) {
-        ring[ri] = (RT)0;
-        x_p[i] = xs;
-        s_p[i] = s_new;
+        store_wt<!LEAN>(&ring[ri], (RT)0);
+        store_wt<!LEAN>(&x_p[i], xs);
+        store_wt<!LEAN>(&s_p[i], s_new);
         if (spiked) spike_marks<LEAN>(a, nr, i, t, tl);
     }
     if (lane == 0) spike_words(a, wave, t, rast_origin, mask);
@@ -823,9 +846,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
         }
         // this step's stores (the pushes below never touch slot t)
         if (act) {
-            ring[ri] = (RT)0;
-            a.x[i] = xs;
-            a.s[i] = s_new;
+            store_wt(&ring[ri], (RT)0);
+            store_wt((float2 *)&a.x[i], xs);
+            store_wt((float *)&a.s[i], s_new);
             if (spiked) spike_marks<false>(a, nr, i, t, tl);
         }
         if (lane == 0) spike_words(a, wave, t, rast_origin, mask);
@@ -1159,11 +1182,11 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     // polled has waited for all its earlier memory operations already (the
     // poll's wait); one with an empty in-table waits here.
     if (pl_lo == pl_hi) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-    if (act0) ring[r0] = (RT)0;
-    if (act1) ring[r1] = (RT)0;
+    if (act0) store_wt(&ring[r0], (RT)0);
+    if (act1) store_wt(&ring[r1], (RT)0);
     if (act0 || act1) {
-        x_p[i] = xs;
-        s_p[i] = s;
+        store_wt(&x_p[i], xs);
+        store_wt(&s_p[i], s);
     }
     if (sp1) spike_marks<LEAN>(a, nr, i, t1, tl1);
     if (lane == 0) spike_words(a, wave, t1, rast_origin, m1);
@@ -1414,8 +1437,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_run(StepArgs a) {
         }
     }
     if (act) {
-        a.x[i] = xs;
-        a.s[i] = s;
+        store_wt((float2 *)&a.x[i], xs);
+        store_wt((float *)&a.s[i], s);
     }
 }
 

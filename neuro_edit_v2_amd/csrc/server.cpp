@@ -614,7 +614,7 @@ void worker_loop(nh_server *s) {
         // one scheduling round: every in-flight request advances together
         const uint64_t k = s->ar.graph_steps ? s->ar.graph_steps : s->cfg.chunk_steps;
         int rc = s->ar.run(k);
-        if (!rc) rc = s->ar.sync();
+        if (!rc) rc = s->ar.sync_timed();
         lk.lock();
         if (rc) {
             s->worker_error = rc;

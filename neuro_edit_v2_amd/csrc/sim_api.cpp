@@ -197,14 +197,14 @@ uint64_t nh_dev_mem(const nh_sim *s) { return s ? s->ar.dev_bytes : 0; }
 
 int nh_last_run_ms(nh_sim *s, double *ms) {
     CHECK_SIM(s);
-    TRY(s->ar.sync());
+    TRY(s->ar.sync_timed());
     *ms = s->ar.last_run_ms;
     return NH_OK;
 }
 
 int nh_last_run_tail(nh_sim *s, double *ms, uint64_t *launches) {
     CHECK_SIM(s);
-    TRY(s->ar.sync());
+    TRY(s->ar.sync_timed());
     *ms = s->ar.last_tail_ms;
     *launches = s->ar.last_tail_launches;
     return NH_OK;
