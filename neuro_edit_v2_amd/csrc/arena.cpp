@@ -483,7 +483,7 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
 
     // ---- per-network records, per-wave network index
     recs.resize(nn);
-    std::vector<uint16_t> wave_net(std::max<uint32_t>(words, 1), 0);
+    std::vector<uint32_t> wave_net(std::max<uint32_t>(words, 1), 0);
     for (uint32_t k = 0; k < nn; ++k) {
         const nh_config &c = cfgs[k];
         NetRec &r = recs[k];
@@ -497,7 +497,7 @@ int Arena::build(uint32_t nn, const nh_net_desc *nets, const nh_config *cfgs, ui
         r.stim_base = slots[k].stim_base;
         r.active = 1;
         r.dt = c.dt_ms;
-        for (uint32_t w = r.base / 64; w < (r.base + align64(r.n)) / 64; ++w) wave_net[w] = (uint16_t)k;
+        for (uint32_t w = r.base / 64; w < (r.base + align64(r.n)) / 64; ++w) wave_net[w] = k;
     }
 
     // ---- initial state and stimulus map over arena ids
@@ -663,7 +663,7 @@ StepArgs Arena::args() const {
     a.k = 0;
     a.slot = 0;
     a.single_net = (n_nets == 1 && !dynamic) ? 1 : 0;
-    a.wave_net = d_wave_net.as<uint16_t>();
+    a.wave_net = d_wave_net.as<uint32_t>();
     a.nets = d_nets.as<NetRec>();
     a.net0 = recs[0];
     a.x = d_x.as<float2>();

@@ -650,7 +650,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, fl
     const float s_old = s_p[i];
     float2 xs = x_p[i];
     uint32_t prow = 0;
-    if (PMODE != 1 && prow_p) prow = prow_p[i];
+    if (PMODE != 1) {
+        // parameter-row index, loaded without a branch (where there is no
+        // index array a lane reads a byte of its own state and ignores it):
+        // a load inside a branch is waited for where the branch ends, and
+        // with it every load issued before it
+        const uint8_t *pb = prow_p ? prow_p + i : (const uint8_t *)(x_p + i);
+        const uint32_t r = *pb;
+        prow = prow_p ? r : 0u;
+    }
     // The network record is copied by value, so its fields are read once,
     // up front, into scalar registers (a reference that may point at either
     // the kernel arguments or the table makes every field a separate vector
@@ -660,7 +668,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step(void *ring_p, float *s_p, fl
     if (LEAN || a.single_net) {
         nr = a.net0;
     } else {
-        net = uniform((uint32_t)a.wave_net[wave]);
+        net = sload(a.wave_net + wave);
         nr = sload(a.nets + net);
     }
     const float ext = a.ext ? a.ext[i] : 0.0f;
@@ -792,7 +800,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_step_binned(StepArgs a) {
         if (a.single_net) {
             nr = a.net0;
         } else {
-            net = uniform((uint32_t)a.wave_net[wave]);
+            net = sload(a.wave_net + wave);
             nr = sload(a.nets + net);
         }
         RT *ring = (RT *)a.ring;
@@ -1031,13 +1039,21 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     float s = s_p[i];
     float2 xs = x_p[i];
     uint32_t prow = 0;
-    if (PMODE != 1 && prow_p) prow = prow_p[i];
+    if (PMODE != 1) {
+        // parameter-row index, loaded without a branch (where there is no
+        // index array a lane reads a byte of its own state and ignores it):
+        // a load inside a branch is waited for where the branch ends, and
+        // with it every load issued before it
+        const uint8_t *pb = prow_p ? prow_p + i : (const uint8_t *)(x_p + i);
+        const uint32_t r = *pb;
+        prow = prow_p ? r : 0u;
+    }
     NetRec nr;
     uint32_t net = 0;
     if (LEAN || a.single_net) {
         nr = a.net0;
     } else {
-        net = uniform((uint32_t)a.wave_net[wave]);
+        net = sload(a.wave_net + wave);
         nr = sload(a.nets + net);
     }
     const float ext = a.ext ? a.ext[i] : 0.0f;

@@ -98,7 +98,7 @@ struct StepArgs {
     uint32_t k;                  // step offset within the chunk
     uint32_t slot;               // (t_dev + k) % D, fixed by the launcher
     uint32_t single_net;         // 1: every wave uses net0 (no table read)
-    gptr<const uint16_t> wave_net;    // per wave: index into nets
+    gptr<const uint32_t> wave_net;    // per wave: index into nets (a word: read with a scalar load)
     gptr<const NetRec> nets;
     NetRec net0;                 // the network of a single-network arena
     // state

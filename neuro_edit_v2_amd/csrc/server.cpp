@@ -291,7 +291,7 @@ int build_arena(nh_server *s) {
     TRY(A(ar.d_t, 8));
     TRY(A(ar.d_nets, (uint64_t)c.max_requests * sizeof(nh::NetRec)));
     HIPCHK(hipMemcpy(ar.d_nets.p, ar.recs.data(), ar.d_nets.bytes, hipMemcpyHostToDevice));
-    TRY(A(ar.d_wave_net, (uint64_t)ar.words * 2));
+    TRY(A(ar.d_wave_net, (uint64_t)ar.words * 4));
     TRY(A(ar.d_x, n * 8));
     TRY(A(ar.d_s, n * 4));
     TRY(A(ar.d_ring, (uint64_t)ar.D * n * 4));
@@ -440,8 +440,8 @@ int admit(nh_server *s, Request *r) {
     std::fill(x0, x0 + na, make_float2(0.f, 0.f));
     std::copy(r->x0.begin(), r->x0.end(), x0);
     TRY(put(ar.d_x, b, x0, na, st));
-    uint16_t *wn = sg.take<uint16_t>(na / 64);
-    std::fill(wn, wn + na / 64, (uint16_t)r->slot);
+    uint32_t *wn = sg.take<uint32_t>(na / 64);
+    std::fill(wn, wn + na / 64, (uint32_t)r->slot);
     TRY(put(ar.d_wave_net, b / 64, wn, na / 64, st));
     if (s->pairs) {
         // delay-1 in-table: per-wave ranges in the pool, sources to arena ids
