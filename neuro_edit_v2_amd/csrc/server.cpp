@@ -280,8 +280,8 @@ int build_arena(nh_server *s) {
     ar.slots.resize(c.max_requests);
     HIPCHK(hipGetDevice(&ar.device));
     HIPCHK(hipStreamCreateWithFlags(&ar.stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&ar.ev0));
-    HIPCHK(hipEventCreate(&ar.ev1));
+    HIPCHK(hipEventCreateWithFlags(&ar.ev0, hipEventDisableSystemFence));
+    HIPCHK(hipEventCreateWithFlags(&ar.ev1, hipEventDisableSystemFence));
     const uint64_t n = ar.n_total;
     auto A = [&](nh::DBuf &b, uint64_t bytes) -> int {
         HIPCHK(b.alloc(bytes));
