@@ -23,7 +23,7 @@
 //    resident at once; the delay-1 arrivals of the first step are pulled by
 //    their targets' waves from published spike words (see its comment).
 //    k_pair8 is the same code built for 8 waves per SIMD, for one network
-//    whose grid is too large for k_pair's 7.
+//    whose grid is too large for k_pair's 6.
 //
 // The int ring (sim/src/kernels.rs:57-66) sums integers, which commute, so
 // every shape gives the CPU raster bit for bit whatever the atomic order.
@@ -1216,12 +1216,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(void *ring, float *s, float2
                                                        arg_block(av));
 }
 
-// The same kernel held to the register budget of 8 waves per SIMD (the
-// default build's 98 scalar registers allow 7): one network, used only where
-// k_pair's grid does not fit on the device at once but this one's does
-// (measured: 250k neurons 5.31 -> 3.99 us per step against one step per
-// launch; at 100k the spills it takes cost 0.14 us per step, so k_pair runs
-// there).
+// The same kernel without the speculative second step, held to the
+// register budget of 8 waves per SIMD (the default build's 73 vector
+// registers allow 6): one network, used only where k_pair's grid does not
+// fit on the device at once but this one's does (measured when it was added:
+// 250k neurons 5.31 -> 3.99 us per step against one step per launch; at 100k
+// the spills it took cost 0.14 us per step, so k_pair runs there).
 template <int RING, int PMODE, int MMODE, bool PACKED>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pair8(
     void *ring, float *s, float2 *x, uint32_t slots, uint32_t n_total, uint32_t words_wpb, uint32_t k,
@@ -1268,7 +1268,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8))) 
 // Every wait is bounded as in k_pair (pair_spins polls, then the abort
 // flag, which every other wait also checks): the launch always ends.
 // Measured on the 100k network: 4.06 us per step over 2000-step runs and
-// 4.2 us per step over 20-step runs, against 2.87 and 3.34 for k_pair --
+// 4.2 us per step over 20-step runs, against 2.87 and 3.34 for k_pair at
+// the time (2.27 and 3.3 since) --
 // each step here chains about five memory round trips across XCDs (gate,
 // ring read, pull, push, completion) where a pair of steps chains about
 // four plus a launch -- so it runs only when asked for (step shape 3).
