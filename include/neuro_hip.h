@@ -219,8 +219,9 @@ int nh_last_run_ms(nh_sim *s, double *ms);
 /* The last run's time after its first step launch (addition; no
  * counterpart in the reference): milliseconds from the end of its first
  * step launch (or first graph replay) to the end of the run (which holds
- * the chunks' one-thread step-counter advance launches as well), and the
- * step launches in that span (a paired launch counts once).  This is the
+ * any one-thread step-counter advance launches as well: chunks with
+ * stimulus inputs and graph replays have them), and the step launches in
+ * that span (a paired launch counts once).  This is the
  * back-to-back time per launch without the host's latency before the first
  * kernel.  *launches is 0 (and *ms the whole run's time) when the run had
  * no launch after its first. */
