@@ -322,15 +322,39 @@ __device__ __forceinline__ void list_batch(PushList &pl, uint32_t base, uint32_t
     }
 }
 
-// Push the list's batches from position `from` on.
-template <int RING, bool PACKED, bool SKIP1 = false>
+// Push the list's batches from position `from` on.  PIPE: pipelined, the
+// next batch's loads issued before the current batch's atomics (a wave's
+// loads complete in order, so waiting for the current batch does not wait
+// for the next one's).  Measured: the paired kernel in the reference's
+// bench regime (≈ 500 deliveries per wave and step) 6.34 -> 5.91 us per
+// step; the batched one-step kernel 1.5 % slower, so it keeps the plain loop.
+template <int RING, bool PACKED, bool SKIP1 = false, bool PIPE = false>
 __device__ __forceinline__ void push_list_from(const StepArgs &a, PushList &pl, uint32_t from,
                                                uint32_t row_lo, uint32_t row_hi, uint32_t slot0,
                                                uint32_t lane) {
-    for (uint32_t base = from; base < pl.total; base += kPushUnroll * kWave) {
-        uint32_t j[kPushUnroll];
-        list_batch(pl, base, row_lo, row_hi, lane, j);
-        deliver_batch<RING, PACKED, SKIP1>(a, j, slot0);
+    using RT = typename Ring<RING>::T;
+    if constexpr (!PIPE) {
+        for (uint32_t base = from; base < pl.total; base += kPushUnroll * kWave) {
+            uint32_t j[kPushUnroll];
+            list_batch(pl, base, row_lo, row_hi, lane, j);
+            deliver_batch<RING, PACKED, SKIP1>(a, j, slot0);
+        }
+    } else {
+        if (from >= pl.total) return;
+        PushBatch<RT> cur;
+        list_batch(pl, from, row_lo, row_hi, lane, cur.j);
+        load_batch<RING, PACKED>(a, cur);
+        for (uint32_t base = from + kPushUnroll * kWave;; base += kPushUnroll * kWave) {
+            const bool more = base < pl.total;
+            PushBatch<RT> nxt;
+            if (more) {
+                list_batch(pl, base, row_lo, row_hi, lane, nxt.j);
+                load_batch<RING, PACKED>(a, nxt);
+            }
+            send_batch<RING, PACKED, SKIP1>(a, cur, slot0);
+            if (!more) break;
+            cur = nxt;
+        }
     }
 }
 
@@ -355,14 +379,19 @@ __device__ __forceinline__ PushList push_first(uint64_t mask, uint32_t row_lo, u
     return pl;
 }
 
-// Push the rows of every spiking lane of a wave.
-template <int RING, bool PACKED, bool SKIP1 = false>
+// Push the rows of every spiking lane of a wave (PIPE: batches pipelined).
+template <int RING, bool PACKED, bool SKIP1 = false, bool PIPE = false>
 __device__ __forceinline__ void push_wave(const StepArgs &a, uint64_t mask, uint32_t row_lo,
                                           uint32_t row_hi, uint32_t slot0, uint32_t lane) {
-    uint32_t j[kPushUnroll];
-    PushList pl = push_first(mask, row_lo, row_hi, lane, j);
-    deliver_batch<RING, PACKED, SKIP1>(a, j, slot0);
-    push_list_from<RING, PACKED, SKIP1>(a, pl, kPushUnroll * kWave, row_lo, row_hi, slot0, lane);
+    if constexpr (!PIPE) {
+        uint32_t j[kPushUnroll];
+        PushList pl = push_first(mask, row_lo, row_hi, lane, j);
+        deliver_batch<RING, PACKED, SKIP1>(a, j, slot0);
+        push_list_from<RING, PACKED, SKIP1>(a, pl, kPushUnroll * kWave, row_lo, row_hi, slot0, lane);
+    } else {
+        PushList pl = push_list(mask, row_lo, row_hi);
+        push_list_from<RING, PACKED, SKIP1, true>(a, pl, 0u, row_lo, row_hi, slot0, lane);
+    }
 }
 
 // Custom-tau synapses of one spike: arrival marks in the per-synapse ring
@@ -1160,7 +1189,7 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     }
     if (m0) {
         send_batch<RING, PACKED, true>(a, pb0, slot0);
-        push_list_from<RING, PACKED, true>(a, pl0, kPushUnroll * kWave, row_lo, row_hi, slot0, lane);
+        push_list_from<RING, PACKED, true, true>(a, pl0, kPushUnroll * kWave, row_lo, row_hi, slot0, lane);
     }
     if (sp0) spike_marks<LEAN>(a, nr, i, t0, tl0);
     if (lane == 0) spike_words(a, wave, t0, rast_origin, m0);
@@ -1185,9 +1214,9 @@ __device__ __forceinline__ void pair_steps(void *ring_p, float *s_p, float2 *x_p
     if (m1) {
         if (SPEC && m1 == m1_sp) {
             send_batch<RING, PACKED, false>(a, pb1, slot1);
-            push_list_from<RING, PACKED, false>(a, pl1, kPushUnroll * kWave, row_lo, row_hi, slot1, lane);
+            push_list_from<RING, PACKED, false, true>(a, pl1, kPushUnroll * kWave, row_lo, row_hi, slot1, lane);
         } else {
-            push_wave<RING, PACKED, false>(a, m1, row_lo, row_hi, slot1, lane);
+            push_wave<RING, PACKED, false, true>(a, m1, row_lo, row_hi, slot1, lane);
         }
     }
 
