@@ -315,7 +315,7 @@ def settle(sim, steps, total):
     """Clock-settle phase, untimed: the timed call run_steps(steps) repeated
     until at least `total` steps have run.  Over the first ~6000 back-to-back
     steps the device ramps to its steady clocks (100k: 2.85 -> 2.76 us per
-    step, flat after), so the driver's short warm-up alone would time the
+    step, flat after), so a short --warmup alone would time the
     ramp; and the library replays a chunk from a captured graph once it has
     seen that chunk's length before, which the repeats give the timed call.
     The last repeats each end in a synchronisation, as the timed call does,
@@ -405,7 +405,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this backend runs on one GPU; multi-GPU figures are not produced "
-              "(networks are independent units -- see DESIGN.md for the sharding plan)",
+              "(the networks of a batch are independent units and would shard by network)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -428,7 +428,7 @@ def main():
     # the warm-up: reading them on the host here would leave the device idle
     # for a host round trip right before the timed steps (at 1M neurons a
     # 1.2 ms read made the next 400 steps 4-5 % slower: the clocks drop).
-    # Warm-up (the driver's W steps), then a clock-settle phase of its own
+    # Warm-up (--warmup steps), then a clock-settle phase of its own
     # (see settle()).
     sim.run_steps(args.warmup)
     settled = settle(sim, args.steps, args.settle)

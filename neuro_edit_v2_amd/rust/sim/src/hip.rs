@@ -135,7 +135,7 @@ impl HipSim {
         })
     }
 
-    /// Gaussian current sd per step (an addition: BASELINE's noise model).
+    /// Gaussian current sd per step (an addition to the reference's kick noise).
     pub fn set_noise_sd(&mut self, sd: f32) -> anyhow::Result<()> {
         check(unsafe { ffi::nh_set_noise_sd(self.h, sd) })
     }
